@@ -1,0 +1,256 @@
+#!/usr/bin/env python3
+"""Benchmark: merge-path SpMV on the 2M-row power-law matrix, one MI355X.
+
+Workload ("c2_powerlaw_2m"): y = A x, A 2,000,000 x 2,000,000 CSR with ~40M
+nonzeros, Zipf(1.8) row lengths on [1, 200000] scaled to 40M, row 0 forced
+to 200,000; columns uniform and sorted per row; int32 indices; fp32 values
+and x uniform [-1, 1).  Synthetic data (seeded generator), resident in HBM
+before timing.
+
+A step is one loops_spmv(MERGE_PATH_FLAT) call (main kernel + row fix-up).
+The timed steps rotate over several identical copies of A (working set >
+the 256 MB MALL) so no step is served from the previous step's cache lines;
+the same-copy (warm) figure is reported beside it.
+
+Prints one JSON line (value = GFLOP/s = 2 nnz / step time) with a roofline
+object for the main merge-path kernel (algorithmic bytes / its average
+duration from HIP events recorded around it on its stream) and the CPU
+baseline (the oracle's sequential validator loop on the host).
+"""
+import argparse
+import json
+import os
+import re
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--copies", type=int, default=4)
+    ap.add_argument("--tpb", type=int, default=0)
+    ap.add_argument("--ipt", type=int, default=0)
+    ap.add_argument("--schedules", default="all",
+                    help="comma list of extra schedules to time, 'all' or 'none'")
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--quick", action="store_true", help="skip extras (profiling runs)")
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the merge-path kernel, measured by rocprofv3
+    PMC runs and recorded in profiles/ (see DESIGN.md); None if absent."""
+    p = os.path.join(ROOT, "profiles", "merge_path_c2_pmc.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build "
+              "does not run multi-GPU jobs.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import edge_loops_amd as loops
+    from edge_loops_amd import synth
+    from edge_loops_amd.hip import Event
+
+    log("generating c2_powerlaw_2m")
+    t0 = time.time()
+    A_h = synth.c2_powerlaw_2m()
+    x_h = synth.uniform(A_h.cols, seed=21)
+    R, C, N = A_h.rows, A_h.cols, A_h.nnz
+    log(f"rows={R} cols={C} nnz={N} max_row={A_h.max_row()} ({time.time()-t0:.1f}s)")
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream().cuda_stream
+    copies = [loops.csr_t.from_host(R, C, A_h.offsets, A_h.indices, A_h.values)
+              for _ in range(max(1, args.copies))]
+    x = torch.from_numpy(x_h).cuda()
+    y = torch.empty(R, dtype=torch.float32, device="cuda")
+    tuning = {}
+    if args.tpb:
+        tuning["threads_per_block"] = args.tpb
+    if args.ipt:
+        tuning["items_per_thread"] = args.ipt
+    ws_bytes = max(loops.spmv_workspace_size("merge_path_flat", copies[0], **tuning), 1)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    nbytes = synth.spmv_bytes(R, C, N, 4)
+    flops = 2 * N
+
+    def step(i, ev=None):
+        A = copies[i % len(copies)]
+        kw = dict(tuning)
+        if ev is not None:
+            kw["start_event"], kw["stop_event"] = ev
+        loops.run_spmv("merge_path_flat", A, x, y, stream=stream, workspace=ws, **kw)
+
+    log(f"warmup {args.warmup}")
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+
+    # Timed region: K steps, events around each main kernel.
+    evs = [(Event(), Event()) for _ in range(args.steps)]
+    t_start, t_stop = Event(), Event()
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for i in range(args.steps):
+        step(i, evs[i])
+    t_stop.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_ms(t_stop)
+    ms_per_step = total_ms / args.steps
+    kern_ms = [a.elapsed_ms(b) for a, b in evs]
+    kern_avg = sum(kern_ms) / len(kern_ms)
+    value = flops / (ms_per_step * 1e-3) / 1e9
+    log(f"cold: {ms_per_step*1e3:.1f} us/step, kernel {kern_avg*1e3:.1f} us, "
+        f"{value:.1f} GFLOP/s, {nbytes/kern_avg/1e6:.0f} GB/s")
+
+    extras = {}
+    if not args.quick:
+        # warm: the same copy every step
+        for i in range(args.warmup):
+            loops.run_spmv("merge_path_flat", copies[0], x, y, stream=stream, workspace=ws, **tuning)
+        w0, w1 = Event(), Event()
+        torch.cuda.synchronize()
+        w0.record(stream)
+        for i in range(args.steps):
+            loops.run_spmv("merge_path_flat", copies[0], x, y, stream=stream, workspace=ws, **tuning)
+        w1.record(stream)
+        torch.cuda.synchronize()
+        warm_ms = w0.elapsed_ms(w1) / args.steps
+        extras["warm_ms_per_step"] = warm_ms
+        extras["warm_gflops"] = flops / (warm_ms * 1e-3) / 1e9
+        log(f"warm: {warm_ms*1e3:.1f} us/step")
+
+        # with precomputed tile coordinates (partition cached per matrix)
+        ti = (args.tpb or 256) * (args.ipt or 8)
+        coords = [loops.merge_path_partition(A, ti, stream=stream) for A in copies]
+        torch.cuda.synchronize()
+        p0, p1 = Event(), Event()
+        p0.record(stream)
+        for i in range(args.steps):
+            loops.run_spmv("merge_path_flat", copies[i % len(copies)], x, y, stream=stream,
+                           workspace=ws, tile_coordinates=coords[i % len(copies)], **tuning)
+        p1.record(stream)
+        torch.cuda.synchronize()
+        pre_ms = p0.elapsed_ms(p1) / args.steps
+        extras["precomputed_partition_ms_per_step"] = pre_ms
+        log(f"precomputed partition: {pre_ms*1e3:.1f} us/step")
+
+        # all five schedules, same matrix (copy 0), a few runs each
+        names = ["merge_path_flat", "work_oriented", "group_mapped", "thread_mapped", "original"]
+        if args.schedules == "none":
+            names = []
+        elif args.schedules != "all":
+            names = args.schedules.split(",")
+        sched = {}
+        for s in names:
+            wsz = max(loops.spmv_workspace_size(s, copies[0]), 1)
+            wss = torch.empty(wsz, dtype=torch.uint8, device="cuda")
+            loops.run_spmv(s, copies[0], x, y, stream=stream, workspace=wss)
+            torch.cuda.synchronize()
+            reps = 10 if s in ("merge_path_flat", "work_oriented") else 3
+            e0, e1 = Event(), Event()
+            e0.record(stream)
+            for _ in range(reps):
+                loops.run_spmv(s, copies[0], x, y, stream=stream, workspace=wss)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_ms(e1) / reps
+            sched[s] = {"ms": round(ms, 5), "gflops": round(flops / (ms * 1e-3) / 1e9, 2),
+                        "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+            log(f"schedule {s}: {ms*1e3:.1f} us")
+        extras["schedules_warm"] = sched
+
+    # CPU baseline leg: the oracle's sequential validator loop
+    # (examples/spmv/helpers.hxx:106-112), timed on this host.
+    import oracle
+    log("cpu baseline")
+    runs = []
+    for _ in range(args.cpu_runs):
+        c0 = time.perf_counter()
+        oracle.spmv_f32(A_h.offsets, A_h.indices, A_h.values, x_h)
+        runs.append(time.perf_counter() - c0)
+    cpu_s = statistics.median(runs)
+    cpu_gflops = flops / cpu_s / 1e9
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    mt = min(16, ncores)
+    runs_mt = []
+    for _ in range(args.cpu_runs):
+        c0 = time.perf_counter()
+        oracle.spmv_f32_mt(A_h.offsets, A_h.indices, A_h.values, x_h, mt)
+        runs_mt.append(time.perf_counter() - c0)
+    cpu_mt_s = statistics.median(runs_mt)
+    log(f"cpu: 1 core {cpu_s*1e3:.1f} ms, {mt} threads {cpu_mt_s*1e3:.1f} ms")
+
+    achieved = nbytes / (kern_avg * 1e-3) / 1e9
+    out = {
+        "metric": "SpMV GFLOP/s + % of HBM roofline; served req/s & p50/p99 vs plain per-call",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "c2_powerlaw_2m merge_path_flat SpMV",
+            "rows": R, "cols": C, "nnz": N, "max_row": A_h.max_row(),
+            "zipf": 1.8, "index": "int32", "values": "f32",
+            "threads_per_block": args.tpb or None, "items_per_thread": args.ipt or None,
+            "rotated_copies": len(copies), "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(),
+            "kernel": "spmv_merge_path_kernel",
+            "kernel_avg_ms": round(kern_avg, 5),
+            "algorithmic_bytes": nbytes,
+        },
+        "cpu_baseline": {
+            "value": round(cpu_gflops, 3),
+            "unit": "GFLOP/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"full c2 matrix ({N} nnz), median of {args.cpu_runs} runs of the "
+                      "oracle's sequential fp32 validator loop",
+            "all_cores": {"threads": mt, "value": round(flops / cpu_mt_s / 1e9, 3)},
+        },
+    }
+    out.update(extras)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+"""Host entry points mirroring the reference's loops::algorithms namespace.
+
+    spmv.original / thread_mapped / group_mapped / work_oriented (csr, x, y)
+        algorithms/spmv/{original,thread_mapped,group_mapped,work_oriented}.cuh
+    spmv.merge_path_flat(csr, x, y) -> timer_t
+        algorithms/spmv/merge_path_flat.cuh:96-139
+    spmm.thread_mapped(csr, B, C)      algorithms/spmm/thread_mapped.cuh:67-90
+    spmm.group_mapped / merge_path     (new: the reference has no such SpMM)
+
+Like the reference, each entry point launches on a stream (default: torch's
+current stream) and synchronises it before returning; ``run_spmv`` /
+``run_spmm`` are the asynchronous forms with explicit tuning and workspace.
+Unlike the reference, y and C are overwritten rather than accumulated into,
+so they need not be zeroed first.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _capi
+from .hip import Event, stream_synchronize, timer_t
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream
+
+
+def _ptr(t):
+    return t.data_ptr() if t is not None and t.numel() else None
+
+
+def make_params(threads_per_block=0, items_per_thread=0, threads_per_tile=0,
+                grid_size=0, tile_coordinates=None, start_event=None,
+                stop_event=None):
+    return _capi.Params(
+        int(threads_per_block), int(items_per_thread), int(threads_per_tile),
+        int(grid_size),
+        _ptr(tile_coordinates) if tile_coordinates is not None else None,
+        start_event.ptr if start_event is not None else None,
+        stop_event.ptr if stop_event is not None else None)
+
+
+def spmv_workspace_size(schedule, csr, **tuning):
+    sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
+    n = ctypes.c_size_t()
+    v = csr.view()
+    p = make_params(**tuning)
+    _capi.check(_capi.lib().loops_spmv_workspace_size(
+        _capi.handle(), sched, ctypes.byref(v), ctypes.byref(p), ctypes.byref(n)),
+        "loops_spmv_workspace_size")
+    return int(n.value)
+
+
+def run_spmv(schedule, csr, x, y, stream=None, workspace=None, **tuning):
+    """Asynchronous y = A x through loops_spmv (no synchronisation)."""
+    sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
+    v = csr.view()
+    p = make_params(**tuning)
+    ws = _ptr(workspace) if workspace is not None else None
+    wsb = workspace.numel() * workspace.element_size() if workspace is not None else 0
+    _capi.check(_capi.lib().loops_spmv(
+        _capi.handle(), sched, ctypes.byref(v), _ptr(x), _ptr(y), ctypes.byref(p),
+        ws, wsb, _stream(stream)), f"loops_spmv({schedule})")
+
+
+def spmm_workspace_size(schedule, csr, k, **tuning):
+    sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
+    n = ctypes.c_size_t()
+    v = csr.view()
+    p = make_params(**tuning)
+    _capi.check(_capi.lib().loops_spmm_workspace_size(
+        _capi.handle(), sched, ctypes.byref(v), int(k), ctypes.byref(p),
+        ctypes.byref(n)), "loops_spmm_workspace_size")
+    return int(n.value)
+
+
+def run_spmm(schedule, csr, B, C, stream=None, workspace=None, **tuning):
+    """Asynchronous C = A B (row-major fp32 B: cols x k, C: rows x k)."""
+    sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
+    k = B.shape[1]
+    v = csr.view()
+    p = make_params(**tuning)
+    ws = _ptr(workspace) if workspace is not None else None
+    wsb = workspace.numel() * workspace.element_size() if workspace is not None else 0
+    _capi.check(_capi.lib().loops_spmm(
+        _capi.handle(), sched, ctypes.byref(v), int(k), _ptr(B), B.stride(0),
+        _ptr(C), C.stride(0), ctypes.byref(p), ws, wsb, _stream(stream)),
+        f"loops_spmm({schedule})")
+
+
+class spmv:
+    """loops::algorithms::spmv."""
+
+    @staticmethod
+    def original(csr, x, y, stream=None, **tuning):
+        s = _stream(stream)
+        run_spmv("original", csr, x, y, s, **tuning)
+        stream_synchronize(s)
+
+    @staticmethod
+    def thread_mapped(csr, x, y, stream=None, **tuning):
+        s = _stream(stream)
+        run_spmv("thread_mapped", csr, x, y, s, **tuning)
+        stream_synchronize(s)
+
+    @staticmethod
+    def group_mapped(csr, x, y, stream=None, **tuning):
+        s = _stream(stream)
+        run_spmv("group_mapped", csr, x, y, s, **tuning)
+        stream_synchronize(s)
+
+    @staticmethod
+    def work_oriented(csr, x, y, stream=None, **tuning):
+        s = _stream(stream)
+        run_spmv("work_oriented", csr, x, y, s, **tuning)
+        stream_synchronize(s)
+
+    @staticmethod
+    def merge_path_flat(csr, x, y, stream=None, **tuning):
+        """Returns a timer_t measuring the merge-path kernel, like the
+        reference (merge_path_flat.cuh:121-138)."""
+        s = _stream(stream)
+        t = timer_t(s)
+        run_spmv("merge_path_flat", csr, x, y, s, start_event=t.start_ev,
+                 stop_event=t.stop_ev, **tuning)
+        stream_synchronize(s)
+        t.time = t.start_ev.elapsed_ms(t.stop_ev)
+        return t
+
+    merge_path = merge_path_flat
+
+
+class spmm:
+    """loops::algorithms::spmm."""
+
+    @staticmethod
+    def thread_mapped(csr, B, C, stream=None, **tuning):
+        s = _stream(stream)
+        run_spmm("thread_mapped", csr, B, C, s, **tuning)
+        stream_synchronize(s)
+
+    @staticmethod
+    def group_mapped(csr, B, C, stream=None, **tuning):
+        s = _stream(stream)
+        run_spmm("group_mapped", csr, B, C, s, **tuning)
+        stream_synchronize(s)
+
+    @staticmethod
+    def merge_path(csr, B, C, stream=None, **tuning):
+        s = _stream(stream)
+        run_spmm("merge_path_flat", csr, B, C, s, **tuning)
+        stream_synchronize(s)
+
+
+def merge_path_partition(csr, items_per_tile, stream=None):
+    """Tile coordinates (num_tiles + 1, 2) as int32 (x = row, y = nonzero):
+    the reference's generate_search_coordinates
+    (schedule/merge_path_flat.hxx:108-138)."""
+    import torch
+    T = ctypes.c_int64()
+    _capi.check(_capi.lib().loops_merge_path_num_tiles(
+        csr.rows, csr.nnzs, int(items_per_tile), ctypes.byref(T)),
+        "loops_merge_path_num_tiles")
+    coords = torch.empty((T.value + 1, 2), dtype=torch.int32, device=csr.offsets.device)
+    _capi.check(_capi.lib().loops_merge_path_partition(
+        _ptr(csr.offsets), csr.rows, csr.nnzs, int(items_per_tile), _ptr(coords),
+        _stream(stream)), "loops_merge_path_partition")
+    return coords
+
+
+def work_oriented_partition(csr, num_threads, stream=None):
+    """Per-thread (start, end) coordinates, shape (num_threads, 4)
+    (schedule/work_oriented.hxx:150-167)."""
+    import torch
+    coords = torch.empty((int(num_threads), 4), dtype=torch.int32, device=csr.offsets.device)
+    _capi.check(_capi.lib().loops_work_oriented_partition(
+        _ptr(csr.offsets), csr.rows, csr.nnzs, int(num_threads), _ptr(coords),
+        _stream(stream)), "loops_work_oriented_partition")
+    return coords
+
+
+def device_cu_count():
+    n = ctypes.c_int32()
+    _capi.check(_capi.lib().loops_device_cu_count(_capi.handle(), ctypes.byref(n)),
+                "loops_device_cu_count")
+    return int(n.value)

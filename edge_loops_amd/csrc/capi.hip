@@ -1,0 +1,362 @@
+// C ABI of libloops_mi355x.so (declared in include/loops_capi.h): argument
+// validation, defaults, workspace handling and dispatch to the kernels.
+// Nothing here computes results on the host; there is no CPU fallback.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+#include <new>
+
+#include "loops_capi.h"
+#include "loops_internal.hpp"
+
+using namespace loops_mi355x;
+
+struct loops_handle {
+  int device = 0;
+  int cu_count = 0;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+namespace {
+
+thread_local int g_last_hip_error = 0;
+
+loops_status_t hip_status(hipError_t e) {
+  if (e == hipSuccess) return LOOPS_SUCCESS;
+  g_last_hip_error = (int)e;
+  if (e == hipErrorNotSupported) return LOOPS_ERROR_NOT_SUPPORTED;
+  return LOOPS_ERROR_HIP;
+}
+
+// Default tunings (see the constants' use below).
+constexpr int kMpDefaultTpb = 256;
+constexpr int kMpDefaultIpt = 8;
+constexpr int kRefBlock = 128;  // reference host entries use 128-thread blocks
+
+loops_status_t check_csr(const loops_csr_view_t* A, csr_args* out) {
+  if (!A) return LOOPS_ERROR_INVALID_VALUE;
+  if (A->rows < 0 || A->cols < 0 || A->nnzs < 0) return LOOPS_ERROR_INVALID_VALUE;
+  if (A->rows + A->nnzs >= (int64_t)INT_MAX || A->cols >= (int64_t)INT_MAX)
+    return LOOPS_ERROR_OVERFLOW;
+  if (A->value_type != LOOPS_VALUE_F32 && A->value_type != LOOPS_VALUE_BF16 &&
+      A->value_type != LOOPS_VALUE_F16)
+    return LOOPS_ERROR_INVALID_VALUE;
+  if (A->rows > 0 && !A->offsets) return LOOPS_ERROR_INVALID_VALUE;
+  if (A->nnzs > 0 && (!A->indices || !A->values)) return LOOPS_ERROR_INVALID_VALUE;
+  out->offsets = A->offsets;
+  out->indices = A->indices;
+  out->values = A->values;
+  out->vtype = A->value_type;
+  out->rows = (int)A->rows;
+  out->cols = (int)A->cols;
+  out->nnz = (int)A->nnzs;
+  const uintptr_t vi = reinterpret_cast<uintptr_t>(A->indices);
+  const uintptr_t vv = reinterpret_cast<uintptr_t>(A->values);
+  const uintptr_t va = A->value_type == LOOPS_VALUE_F32 ? 16 : 8;
+  out->aligned = (vi % 16 == 0) && (vv % va == 0);
+  return LOOPS_SUCCESS;
+}
+
+struct resolved {
+  int tpb = 0, ipt = 0, tpt = 0, grid = 0;
+  long long units = 0;   // fix-up units (merge tiles / work_oriented threads)
+  size_t ws_bytes = 0;
+};
+
+loops_status_t resolve_spmv(loops_handle_t h, loops_schedule_t s,
+                            const csr_args& A, const loops_params_t* p,
+                            resolved* r) {
+  const loops_params_t zero{};
+  if (!p) p = &zero;
+  if (p->threads_per_block < 0 || p->items_per_thread < 0 ||
+      p->threads_per_tile < 0 || p->grid_size < 0)
+    return LOOPS_ERROR_INVALID_VALUE;
+  const long long total = (long long)A.rows + A.nnz;
+  switch (s) {
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT: {
+      r->tpb = p->threads_per_block ? p->threads_per_block : kMpDefaultTpb;
+      r->ipt = p->items_per_thread ? p->items_per_thread : kMpDefaultIpt;
+      if (!mp_spmv_supported(r->tpb, r->ipt)) return LOOPS_ERROR_NOT_SUPPORTED;
+      const long long ti = (long long)r->tpb * r->ipt;
+      r->units = (total + ti - 1) / ti;
+      r->ws_bytes = r->units > 0 ? fixup_ws_bytes(r->units) : 0;
+      return LOOPS_SUCCESS;
+    }
+    case LOOPS_SCHEDULE_WORK_ORIENTED: {
+      r->tpb = p->threads_per_block ? p->threads_per_block : kRefBlock;
+      if (!wo_spmv_supported(r->tpb)) return LOOPS_ERROR_NOT_SUPPORTED;
+      r->grid = p->grid_size ? p->grid_size : 2 * (h ? h->cu_count : 256);
+      r->units = (long long)r->grid * r->tpb;
+      r->ws_bytes = total > 0 ? fixup_ws_bytes(r->units) : 0;
+      return LOOPS_SUCCESS;
+    }
+    case LOOPS_SCHEDULE_GROUP_MAPPED: {
+      r->tpb = p->threads_per_block ? p->threads_per_block : kRefBlock;
+      r->tpt = p->threads_per_tile ? p->threads_per_tile : r->tpb;
+      if (!gm_spmv_supported(r->tpb, r->tpt)) return LOOPS_ERROR_NOT_SUPPORTED;
+      return LOOPS_SUCCESS;
+    }
+    case LOOPS_SCHEDULE_THREAD_MAPPED:
+    case LOOPS_SCHEDULE_ORIGINAL: {
+      r->tpb = p->threads_per_block ? p->threads_per_block : kRefBlock;
+      if (r->tpb > 1024 || r->tpb % 64 != 0) return LOOPS_ERROR_NOT_SUPPORTED;
+      return LOOPS_SUCCESS;
+    }
+    case LOOPS_SCHEDULE_BUCKETING:
+      return LOOPS_ERROR_NOT_SUPPORTED;
+  }
+  return LOOPS_ERROR_INVALID_VALUE;
+}
+
+loops_status_t get_workspace(loops_handle_t h, size_t need, void* ws,
+                             size_t ws_bytes, void** out) {
+  *out = ws;
+  if (need == 0) return LOOPS_SUCCESS;
+  if (ws) return ws_bytes >= need ? LOOPS_SUCCESS : LOOPS_ERROR_WORKSPACE;
+  if (!h) return LOOPS_ERROR_WORKSPACE;
+  if (h->ws_bytes < need) {
+    if (h->ws) {
+      hipError_t e = hipFree(h->ws);
+      h->ws = nullptr;
+      h->ws_bytes = 0;
+      if (e != hipSuccess) return hip_status(e);
+    }
+    size_t sz = need < (size_t(1) << 20) ? (size_t(1) << 20) : need;
+    if (hipMalloc(&h->ws, sz) != hipSuccess) {
+      h->ws = nullptr;
+      return LOOPS_ERROR_ALLOCATION;
+    }
+    h->ws_bytes = sz;
+  }
+  *out = h->ws;
+  return LOOPS_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+loops_status_t loops_create(loops_handle_t* handle) {
+  if (!handle) return LOOPS_ERROR_INVALID_VALUE;
+  loops_handle* h = new (std::nothrow) loops_handle();
+  if (!h) return LOOPS_ERROR_ALLOCATION;
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount,
+                              h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return hip_status(e);
+  }
+  *handle = h;
+  return LOOPS_SUCCESS;
+}
+
+loops_status_t loops_destroy(loops_handle_t handle) {
+  if (!handle) return LOOPS_ERROR_INVALID_VALUE;
+  hipError_t e = hipSuccess;
+  if (handle->ws) e = hipFree(handle->ws);
+  delete handle;
+  return hip_status(e);
+}
+
+const char* loops_status_string(loops_status_t status) {
+  switch (status) {
+    case LOOPS_SUCCESS: return "success";
+    case LOOPS_ERROR_INVALID_VALUE: return "invalid value";
+    case LOOPS_ERROR_OVERFLOW: return "index overflow (rows + nnzs must be < 2^31)";
+    case LOOPS_ERROR_WORKSPACE: return "workspace missing or too small";
+    case LOOPS_ERROR_NOT_SUPPORTED: return "not supported";
+    case LOOPS_ERROR_HIP: return "HIP runtime error";
+    case LOOPS_ERROR_ALLOCATION: return "allocation failed";
+  }
+  return "unknown status";
+}
+
+const char* loops_schedule_name(loops_schedule_t schedule) {
+  switch (schedule) {
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT: return "merge_path_flat";
+    case LOOPS_SCHEDULE_WORK_ORIENTED: return "work_oriented";
+    case LOOPS_SCHEDULE_THREAD_MAPPED: return "thread_mapped";
+    case LOOPS_SCHEDULE_GROUP_MAPPED: return "group_mapped";
+    case LOOPS_SCHEDULE_BUCKETING: return "bucketing";
+    case LOOPS_SCHEDULE_ORIGINAL: return "original";
+  }
+  return "unknown";
+}
+
+int loops_last_hip_error(void) { return g_last_hip_error; }
+
+loops_status_t loops_device_cu_count(loops_handle_t handle, int32_t* cu_count) {
+  if (!handle || !cu_count) return LOOPS_ERROR_INVALID_VALUE;
+  *cu_count = handle->cu_count;
+  return LOOPS_SUCCESS;
+}
+
+loops_status_t loops_spmv_workspace_size(loops_handle_t handle,
+                                         loops_schedule_t schedule,
+                                         const loops_csr_view_t* A,
+                                         const loops_params_t* params,
+                                         size_t* bytes) {
+  if (!bytes) return LOOPS_ERROR_INVALID_VALUE;
+  csr_args a;
+  loops_status_t st = check_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  resolved r;
+  st = resolve_spmv(handle, schedule, a, params, &r);
+  if (st != LOOPS_SUCCESS) return st;
+  *bytes = r.ws_bytes;
+  return LOOPS_SUCCESS;
+}
+
+loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
+                          const loops_csr_view_t* A, const float* x, float* y,
+                          const loops_params_t* params, void* workspace,
+                          size_t workspace_bytes, hipStream_t stream) {
+  csr_args a;
+  loops_status_t st = check_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  if (a.rows > 0 && !y) return LOOPS_ERROR_INVALID_VALUE;
+  if (a.nnz > 0 && !x) return LOOPS_ERROR_INVALID_VALUE;
+  resolved r;
+  st = resolve_spmv(handle, schedule, a, params, &r);
+  if (st != LOOPS_SUCCESS) return st;
+  void* ws = nullptr;
+  st = get_workspace(handle, r.ws_bytes, workspace, workspace_bytes, &ws);
+  if (st != LOOPS_SUCCESS) return st;
+  hipEvent_t ev0 = params ? params->start_event : nullptr;
+  hipEvent_t ev1 = params ? params->stop_event : nullptr;
+  switch (schedule) {
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT:
+      return hip_status(launch_mp_spmv(a, x, y, r.tpb, r.ipt,
+                                       params ? params->tile_coordinates : nullptr,
+                                       carve_fixup_ws(ws, r.units), ev0, ev1,
+                                       stream));
+    case LOOPS_SCHEDULE_WORK_ORIENTED:
+      return hip_status(launch_wo_spmv(a, x, y, r.tpb, r.grid,
+                                       carve_fixup_ws(ws, r.units), ev0, ev1,
+                                       stream));
+    case LOOPS_SCHEDULE_GROUP_MAPPED:
+      return hip_status(launch_gm_spmv(a, x, y, r.tpb, r.tpt, ev0, ev1, stream));
+    case LOOPS_SCHEDULE_THREAD_MAPPED:
+      return hip_status(launch_tm_spmv(a, x, y, r.tpb, false, ev0, ev1, stream));
+    case LOOPS_SCHEDULE_ORIGINAL:
+      return hip_status(launch_tm_spmv(a, x, y, r.tpb, true, ev0, ev1, stream));
+    default:
+      return LOOPS_ERROR_NOT_SUPPORTED;
+  }
+}
+
+loops_status_t loops_merge_path_num_tiles(int64_t rows, int64_t nnzs,
+                                          int64_t items_per_tile,
+                                          int64_t* num_tiles) {
+  if (!num_tiles || rows < 0 || nnzs < 0 || items_per_tile <= 0)
+    return LOOPS_ERROR_INVALID_VALUE;
+  if (rows + nnzs >= (int64_t)INT_MAX) return LOOPS_ERROR_OVERFLOW;
+  *num_tiles = (rows + nnzs + items_per_tile - 1) / items_per_tile;
+  return LOOPS_SUCCESS;
+}
+
+loops_status_t loops_merge_path_partition(const int32_t* offsets, int64_t rows,
+                                          int64_t nnzs, int64_t items_per_tile,
+                                          loops_coord_t* coords,
+                                          hipStream_t stream) {
+  int64_t tiles = 0;
+  loops_status_t st = loops_merge_path_num_tiles(rows, nnzs, items_per_tile, &tiles);
+  if (st != LOOPS_SUCCESS) return st;
+  if (!coords || (rows > 0 && !offsets)) return LOOPS_ERROR_INVALID_VALUE;
+  return hip_status(launch_mp_partition(offsets, (int)rows, (int)nnzs,
+                                        items_per_tile, tiles + 1, coords, stream));
+}
+
+loops_status_t loops_work_oriented_partition(const int32_t* offsets,
+                                             int64_t rows, int64_t nnzs,
+                                             int64_t num_threads,
+                                             loops_coord_t* coords,
+                                             hipStream_t stream) {
+  if (rows < 0 || nnzs < 0 || num_threads <= 0 || !coords ||
+      (rows > 0 && !offsets))
+    return LOOPS_ERROR_INVALID_VALUE;
+  if (rows + nnzs >= (int64_t)INT_MAX) return LOOPS_ERROR_OVERFLOW;
+  return hip_status(launch_wo_partition(offsets, (int)rows, (int)nnzs,
+                                        num_threads, coords, stream));
+}
+
+loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
+                                         loops_schedule_t schedule,
+                                         const loops_csr_view_t* A, int64_t k,
+                                         const loops_params_t* params,
+                                         size_t* bytes) {
+  (void)handle;
+  if (!bytes || k < 0 || k > INT_MAX) return LOOPS_ERROR_INVALID_VALUE;
+  csr_args a;
+  loops_status_t st = check_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  const loops_params_t zero{};
+  if (!params) params = &zero;
+  switch (schedule) {
+    case LOOPS_SCHEDULE_THREAD_MAPPED:
+      *bytes = 0;
+      return LOOPS_SUCCESS;
+    case LOOPS_SCHEDULE_GROUP_MAPPED: {
+      int tpb = params->threads_per_block ? params->threads_per_block : 256;
+      if (!gm_spmm_supported(tpb)) return LOOPS_ERROR_NOT_SUPPORTED;
+      *bytes = 0;
+      return LOOPS_SUCCESS;
+    }
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT: {
+      int tpb = params->threads_per_block ? params->threads_per_block : 256;
+      int ipw = params->items_per_thread ? params->items_per_thread : 0;
+      if (!mp_spmm_supported(tpb, ipw)) return LOOPS_ERROR_NOT_SUPPORTED;
+      *bytes = mp_spmm_ws_bytes(mp_spmm_units(a, tpb, ipw), (int)k);
+      return LOOPS_SUCCESS;
+    }
+    default:
+      return LOOPS_ERROR_NOT_SUPPORTED;
+  }
+}
+
+loops_status_t loops_spmm(loops_handle_t handle, loops_schedule_t schedule,
+                          const loops_csr_view_t* A, int64_t k, const float* B,
+                          int64_t ldb, float* C, int64_t ldc,
+                          const loops_params_t* params, void* workspace,
+                          size_t workspace_bytes, hipStream_t stream) {
+  csr_args a;
+  loops_status_t st = check_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  if (k < 0 || k > INT_MAX || ldb < k || ldc < k) return LOOPS_ERROR_INVALID_VALUE;
+  if (k > 0 && ((a.rows > 0 && !C) || (a.nnz > 0 && !B)))
+    return LOOPS_ERROR_INVALID_VALUE;
+  const loops_params_t zero{};
+  const loops_params_t* p = params ? params : &zero;
+  hipEvent_t ev0 = p->start_event, ev1 = p->stop_event;
+  switch (schedule) {
+    case LOOPS_SCHEDULE_THREAD_MAPPED: {
+      int tpb = p->threads_per_block ? p->threads_per_block : kRefBlock;
+      if (tpb > 1024 || tpb % 64 != 0) return LOOPS_ERROR_NOT_SUPPORTED;
+      return hip_status(launch_tm_spmm(a, (int)k, B, ldb, C, ldc, tpb, ev0, ev1, stream));
+    }
+    case LOOPS_SCHEDULE_GROUP_MAPPED: {
+      int tpb = p->threads_per_block ? p->threads_per_block : 256;
+      if (!gm_spmm_supported(tpb)) return LOOPS_ERROR_NOT_SUPPORTED;
+      return hip_status(launch_gm_spmm(a, (int)k, B, ldb, C, ldc, tpb, ev0, ev1, stream));
+    }
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT: {
+      int tpb = p->threads_per_block ? p->threads_per_block : 256;
+      int ipw = p->items_per_thread ? p->items_per_thread : 0;
+      if (!mp_spmm_supported(tpb, ipw)) return LOOPS_ERROR_NOT_SUPPORTED;
+      size_t need = mp_spmm_ws_bytes(mp_spmm_units(a, tpb, ipw), (int)k);
+      void* ws = nullptr;
+      st = get_workspace(handle, need, workspace, workspace_bytes, &ws);
+      if (st != LOOPS_SUCCESS) return st;
+      return hip_status(launch_mp_spmm(a, (int)k, B, ldb, C, ldc, tpb, ipw, ws,
+                                       ev0, ev1, stream));
+    }
+    default:
+      return LOOPS_ERROR_NOT_SUPPORTED;
+  }
+}
+
+}  // extern "C"

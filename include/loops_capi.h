@@ -1,0 +1,187 @@
+/*
+ * loops_capi.h -- C ABI of the MI355X (gfx950) load-balanced CSR SpMV/SpMM
+ * library, libloops_mi355x.so.
+ *
+ * Plain pointers and sizes only.  Every compute entry point is asynchronous
+ * on the hipStream_t it is given and performs no device allocation when the
+ * caller passes a workspace, so it can be captured into a hipGraph.  Status
+ * codes are returned, never thrown.  The C++ drop-in headers under
+ * include/loops/ restore the reference's synchronous, exception-throwing
+ * semantics on top of this ABI.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   loops_spmv(MERGE_PATH_FLAT)   algorithms/spmv/merge_path_flat.cuh:96-139
+ *   loops_spmv(WORK_ORIENTED)     algorithms/spmv/work_oriented.cuh:91-113
+ *   loops_spmv(THREAD_MAPPED)     algorithms/spmv/thread_mapped.cuh:68-91
+ *   loops_spmv(GROUP_MAPPED)      algorithms/spmv/group_mapped.cuh:73-105
+ *   loops_spmv(ORIGINAL)          algorithms/spmv/original.cuh:59-76
+ *   loops_spmm(...)               algorithms/spmm/thread_mapped.cuh:67-90
+ *   loops_merge_path_partition    schedule/merge_path_flat.hxx:108-138 and
+ *                                 the preprocess_t constructor, :172-206
+ *   loops_schedule_t              schedule.hxx:23-29 (same numbering)
+ *
+ * Contract differences from the reference (all deliberate):
+ *   - y (or C) is overwritten, never accumulated into, for every schedule.
+ *     The reference's atomic schedules require y == 0 on entry and add into
+ *     it; with y == 0 both give the same result.
+ *   - Results are deterministic: no schedule uses floating-point atomics.
+ *     Rows split across work units are combined by a fixed-order fix-up.
+ *   - Index and offset types are int32, as in every reference example
+ *     (examples/spmv/merge_path.cu:18-20); rows + nnzs must be < 2^31.
+ */
+#ifndef LOOPS_CAPI_H_
+#define LOOPS_CAPI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LOOPS_CAPI_VERSION 1
+
+/* error.hxx:15-45 (exception_t / throw_if_exception) become status codes. */
+typedef enum loops_status_t {
+  LOOPS_SUCCESS = 0,
+  LOOPS_ERROR_INVALID_VALUE = 1,   /* null pointer, negative size, bad enum */
+  LOOPS_ERROR_OVERFLOW = 2,        /* rows + nnzs >= 2^31 (int32 offsets)   */
+  LOOPS_ERROR_WORKSPACE = 3,       /* workspace missing or too small        */
+  LOOPS_ERROR_NOT_SUPPORTED = 4,   /* schedule / tuning not available       */
+  LOOPS_ERROR_HIP = 5,             /* a HIP runtime call failed             */
+  LOOPS_ERROR_ALLOCATION = 6       /* handle-owned workspace growth failed  */
+} loops_status_t;
+
+/* schedule.hxx:23-29 keeps its numbering; ORIGINAL (algorithms/spmv/
+ * original.cuh, no schedule object) is appended. */
+typedef enum loops_schedule_t {
+  LOOPS_SCHEDULE_MERGE_PATH_FLAT = 0,
+  LOOPS_SCHEDULE_WORK_ORIENTED = 1,
+  LOOPS_SCHEDULE_THREAD_MAPPED = 2,
+  LOOPS_SCHEDULE_GROUP_MAPPED = 3,
+  LOOPS_SCHEDULE_BUCKETING = 4, /* declared by the reference, never built */
+  LOOPS_SCHEDULE_ORIGINAL = 5
+} loops_schedule_t;
+
+/* Value storage of A.  Accumulation is always fp32; x, y, B and C are fp32. */
+typedef enum loops_value_type_t {
+  LOOPS_VALUE_F32 = 0,
+  LOOPS_VALUE_BF16 = 1,
+  LOOPS_VALUE_F16 = 2
+} loops_value_type_t;
+
+/* container/csr.hxx:36-43 as a non-owning view (device pointers). */
+typedef struct loops_csr_view_t {
+  int64_t rows;
+  int64_t cols;
+  int64_t nnzs;
+  const int32_t* offsets; /* rows + 1 entries, offsets[0] == 0           */
+  const int32_t* indices; /* nnzs column indices                          */
+  const void* values;     /* nnzs values of value_type                    */
+  loops_value_type_t value_type;
+} loops_csr_view_t;
+
+/* container/coordinate.hxx:14-18 with coord_t = coordinate_t<unsigned>
+ * (schedule/merge_path_flat.hxx:30): x = row, y = nonzero. */
+typedef struct loops_coord_t {
+  uint32_t x;
+  uint32_t y;
+} loops_coord_t;
+
+/* Launch tuning; a zero field selects the schedule's default.
+ * Defaults follow the reference's host entries:
+ *   merge_path_flat: 128 threads x 5 items (merge_path_flat.cuh:102-103)
+ *   work_oriented:   128 threads, grid 2 x CU count (work_oriented.cuh:97-104)
+ *   group_mapped:    block_mapped<128> (group_mapped.cuh:39,79)
+ *   thread_mapped / original: 128 threads, ceil(rows/128) blocks. */
+typedef struct loops_params_t {
+  int32_t threads_per_block;
+  int32_t items_per_thread;  /* merge_path_flat only                       */
+  int32_t threads_per_tile;  /* group_mapped only: 64 (wave) or TPB        */
+  int32_t grid_size;         /* work_oriented only                         */
+  /* merge_path_flat only: precomputed tile coordinates from
+   * loops_merge_path_partition with items_per_tile = TPB * IPT (the
+   * reference's preprocess_t meta).  NULL: each tile searches itself. */
+  const loops_coord_t* tile_coordinates;
+  /* Optional events recorded around the main kernel, the analogue of the
+   * util::timer_t returned by merge_path_flat (merge_path_flat.cuh:121-138). */
+  hipEvent_t start_event;
+  hipEvent_t stop_event;
+} loops_params_t;
+
+typedef struct loops_handle* loops_handle_t;
+
+/* Handle: caches device properties (CU count) and an optional
+ * library-owned workspace used when a call passes workspace == NULL. */
+loops_status_t loops_create(loops_handle_t* handle);
+loops_status_t loops_destroy(loops_handle_t handle);
+const char* loops_status_string(loops_status_t status);
+const char* loops_schedule_name(loops_schedule_t schedule);
+/* The hipError_t of the last LOOPS_ERROR_HIP returned on this thread. */
+int loops_last_hip_error(void);
+
+/* Device properties as the reference's util/device.hxx:23-52 reads them. */
+loops_status_t loops_device_cu_count(loops_handle_t handle, int32_t* cu_count);
+
+/* Bytes of device workspace loops_spmv needs for this schedule, matrix
+ * shape and tuning (0 for thread_mapped, original and group_mapped). */
+loops_status_t loops_spmv_workspace_size(loops_handle_t handle,
+                                         loops_schedule_t schedule,
+                                         const loops_csr_view_t* A,
+                                         const loops_params_t* params,
+                                         size_t* bytes);
+
+/* y = A * x (y overwritten).  workspace may be NULL, in which case the
+ * handle's own workspace is used (grown with hipMalloc outside any graph
+ * capture).  Asynchronous on stream. */
+loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
+                          const loops_csr_view_t* A, const float* x, float* y,
+                          const loops_params_t* params, void* workspace,
+                          size_t workspace_bytes, hipStream_t stream);
+
+/* Number of merge tiles, ceil((rows + nnzs) / items_per_tile)
+ * (merge_path_flat.cuh:116-117). */
+loops_status_t loops_merge_path_num_tiles(int64_t rows, int64_t nnzs,
+                                          int64_t items_per_tile,
+                                          int64_t* num_tiles);
+
+/* coords[t] = merge-path coordinate at diagonal min(t * items_per_tile,
+ * rows + nnzs) for t in [0, num_tiles] (num_tiles + 1 entries): the
+ * reference's generate_search_coordinates (merge_path_flat.hxx:108-138),
+ * bit-exact with its search (util/search.hxx:34-60). */
+loops_status_t loops_merge_path_partition(const int32_t* offsets, int64_t rows,
+                                          int64_t nnzs, int64_t items_per_tile,
+                                          loops_coord_t* coords,
+                                          hipStream_t stream);
+
+/* work_oriented partition: for each of num_threads threads its start and end
+ * coordinate (work_oriented.hxx:150-167), written as 2 coords per thread. */
+loops_status_t loops_work_oriented_partition(const int32_t* offsets,
+                                             int64_t rows, int64_t nnzs,
+                                             int64_t num_threads,
+                                             loops_coord_t* coords,
+                                             hipStream_t stream);
+
+/* SpMM, C = A * B with B (cols x k) and C (rows x k) dense row-major fp32
+ * (container/matrix.cuh:27-35) with leading dimensions ldb, ldc >= k.
+ * Schedules: THREAD_MAPPED (algorithms/spmm/thread_mapped.cuh:28-53),
+ * GROUP_MAPPED and MERGE_PATH_FLAT (new in this library; the reference has
+ * only thread_mapped SpMM). */
+loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
+                                         loops_schedule_t schedule,
+                                         const loops_csr_view_t* A, int64_t k,
+                                         const loops_params_t* params,
+                                         size_t* bytes);
+loops_status_t loops_spmm(loops_handle_t handle, loops_schedule_t schedule,
+                          const loops_csr_view_t* A, int64_t k, const float* B,
+                          int64_t ldb, float* C, int64_t ldc,
+                          const loops_params_t* params, void* workspace,
+                          size_t workspace_bytes, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOOPS_CAPI_H_ */

@@ -1,0 +1,94 @@
+"""CPU-side checks of the boundary: the C-ABI library loads and exports every
+symbol include/loops_capi.h declares, the host-only entry points validate
+their arguments, and the synthetic generator keeps its contract.  No kernel
+is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import edge_loops_amd as loops
+from edge_loops_amd import _capi, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "loops_capi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(loops_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = _declared_symbols()
+    assert len(names) >= 12
+    L = _capi.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_capi.SIGNATURES)
+
+
+def test_status_and_schedule_strings():
+    L = _capi.lib()
+    assert L.loops_status_string(0) == b"success"
+    assert b"overflow" in L.loops_status_string(2)
+    assert L.loops_schedule_name(0) == b"merge_path_flat"
+    assert L.loops_schedule_name(4) == b"bucketing"
+    assert L.loops_schedule_name(5) == b"original"
+
+
+def test_num_tiles_and_validation():
+    L = _capi.lib()
+    T = ctypes.c_int64()
+    assert L.loops_merge_path_num_tiles(4096, 32768, 640, ctypes.byref(T)) == 0
+    assert T.value == 58  # merge_path_flat.cuh:116-117 for config 1
+    assert L.loops_merge_path_num_tiles(2_000_000, 40_000_002, 2048, ctypes.byref(T)) == 0
+    assert T.value == -(-42_000_002 // 2048)
+    assert L.loops_merge_path_num_tiles(-1, 0, 640, ctypes.byref(T)) == _capi.ERROR_INVALID_VALUE
+    assert L.loops_merge_path_num_tiles(1, 0, 0, ctypes.byref(T)) == _capi.ERROR_INVALID_VALUE
+    assert L.loops_merge_path_num_tiles(2**30, 2**30, 640, ctypes.byref(T)) == _capi.ERROR_OVERFLOW
+    # partition / workspace queries reject bad views before touching a GPU
+    assert L.loops_merge_path_partition(None, 10, 0, 640, None, None) == _capi.ERROR_INVALID_VALUE
+    v = _capi.CsrView(-1, 4, 0, None, None, None, 0)
+    n = ctypes.c_size_t()
+    assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_INVALID_VALUE
+    v = _capi.CsrView(2**30, 4, 2**30, 1, 1, 1, 0)
+    assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_OVERFLOW
+    v = _capi.CsrView(4, 4, 4, 1, 1, 1, 0)
+    assert L.loops_spmv_workspace_size(None, 4, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_NOT_SUPPORTED
+    p = _capi.Params(128, 7, 0, 0, None, None, None)  # no (128, 7) tuning
+    assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), ctypes.byref(p), ctypes.byref(n)) == _capi.ERROR_NOT_SUPPORTED
+    p = _capi.Params(128, 5, 0, 0, None, None, None)
+    assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), ctypes.byref(p), ctypes.byref(n)) == 0
+    assert n.value >= 12  # one merge tile: carry row + two partial sums
+    p = _capi.Params(0, 0, 0, 0, None, None, None)
+    assert L.loops_spmv_workspace_size(None, 2, ctypes.byref(v), ctypes.byref(p), ctypes.byref(n)) == 0
+    assert n.value == 0  # thread_mapped needs none
+    # null vectors with work to do are rejected before any launch
+    assert L.loops_spmv(None, 2, ctypes.byref(v), None, None, None, None, 0, None) == _capi.ERROR_INVALID_VALUE
+
+
+def test_synth_contract():
+    A = synth.powerlaw_csr(5000, 3000, 1.8, 1, 2000, 40000, seed=5, force_first_max=True)
+    L = np.diff(A.offsets)
+    assert L[0] == 2000 and L.min() >= 1 and L.max() <= 2000
+    assert abs(A.nnz - 40000) < 40000 * 0.01
+    for r in range(0, 5000, 37):
+        seg = A.indices[A.offsets[r]:A.offsets[r + 1]]
+        assert np.all(np.diff(seg) > 0) and seg.min() >= 0 and seg.max() < 3000
+    assert A.values.min() >= -1 and A.values.max() < 1
+    B = synth.powerlaw_csr(5000, 3000, 1.8, 1, 2000, 40000, seed=5, force_first_max=True, nthreads=1)
+    assert np.array_equal(A.indices, B.indices) and np.array_equal(A.values, B.values)
+    # dense rows (selection sampling path)
+    D = synth.powerlaw_csr(10, 16, 0.0, 16, 16, 0, seed=1)
+    assert np.array_equal(D.indices.reshape(10, 16), np.tile(np.arange(16), (10, 1)))
+
+
+def test_python_mirror_surface():
+    for name in ("original", "thread_mapped", "group_mapped", "work_oriented",
+                 "merge_path_flat", "merge_path"):
+        assert callable(getattr(loops.spmv, name))
+    for name in ("thread_mapped", "group_mapped", "merge_path"):
+        assert callable(getattr(loops.spmm, name))

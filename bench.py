@@ -92,8 +92,7 @@ def main():
         tuning["threads_per_block"] = args.tpb
     if args.ipt:
         tuning["items_per_thread"] = args.ipt
-    ws_bytes = max(loops.spmv_workspace_size("merge_path_flat", copies[0], **tuning), 1)
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    ws = loops.algorithms.workspace("merge_path_flat", copies[0], **tuning)
     torch.cuda.synchronize()
 
     nbytes = synth.spmv_bytes(R, C, N, 4)
@@ -168,8 +167,7 @@ def main():
             names = args.schedules.split(",")
         sched = {}
         for s in names:
-            wsz = max(loops.spmv_workspace_size(s, copies[0]), 1)
-            wss = torch.empty(wsz, dtype=torch.uint8, device="cuda")
+            wss = loops.algorithms.workspace(s, copies[0])
             loops.run_spmv(s, copies[0], x, y, stream=stream, workspace=wss)
             torch.cuda.synchronize()
             reps = 10 if s in ("merge_path_flat", "work_oriented") else 3

@@ -5,7 +5,7 @@ csr_t / vector_t / matrix_t) over the C ABI of libloops_mi355x.so
 (include/loops_capi.h).  Device memory and streams come from torch; every
 result is computed by the library's HIP kernels.
 """
-from . import _capi
+from . import _capi, algorithms
 from ._capi import LoopsError, lib
 from .algorithms import (merge_path_partition, run_spmm, run_spmv, spmm,
                          spmm_workspace_size, spmv, spmv_workspace_size,

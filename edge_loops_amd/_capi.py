@@ -72,7 +72,11 @@ class Params(ctypes.Structure):
         ("tile_coordinates", ctypes.c_void_p),
         ("start_event", ctypes.c_void_p),
         ("stop_event", ctypes.c_void_p),
+        ("flags", ctypes.c_uint32),
     ]
+
+
+FLAG_SEPARATE_FIXUP = 1
 
 
 class LoopsError(RuntimeError):

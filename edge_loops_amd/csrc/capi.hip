@@ -17,6 +17,8 @@ struct loops_handle {
   int cu_count = 0;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  int* counters = nullptr;  // in-kernel fix-up counters, kept all-zero
+  long long counter_cap = 0;
 };
 
 namespace {
@@ -134,6 +136,36 @@ loops_status_t get_workspace(loops_handle_t h, size_t need, void* ws,
   return LOOPS_SUCCESS;
 }
 
+// Counters for the in-kernel fix-up of a call with `units` merge tiles, or
+// nullptr when the handle cannot provide them without allocating inside a
+// graph capture (the caller then uses the fix-up kernel instead).
+int* fixup_counters(loops_handle_t h, long long units, hipStream_t s,
+                    loops_status_t* st) {
+  *st = LOOPS_SUCCESS;
+  if (!h) return nullptr;
+  if (h->counter_cap >= units) return h->counters;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return nullptr;
+  long long cap = units < 65536 ? 65536 : units;
+  int* p = nullptr;
+  if (hipMalloc(&p, (size_t)cap * sizeof(int)) != hipSuccess) {
+    *st = LOOPS_ERROR_ALLOCATION;
+    return nullptr;
+  }
+  hipError_t e = hipMemset(p, 0, (size_t)cap * sizeof(int));
+  if (e == hipSuccess) e = hipDeviceSynchronize();  // old counters idle before free
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    *st = hip_status(e);
+    return nullptr;
+  }
+  if (h->counters) (void)hipFree(h->counters);
+  h->counters = p;
+  h->counter_cap = cap;
+  return p;
+}
+
 }  // namespace
 
 extern "C" {
@@ -150,6 +182,12 @@ loops_status_t loops_create(loops_handle_t* handle) {
     delete h;
     return hip_status(e);
   }
+  loops_status_t st;
+  fixup_counters(h, 65536, nullptr, &st);
+  if (st != LOOPS_SUCCESS) {
+    delete h;
+    return st;
+  }
   *handle = h;
   return LOOPS_SUCCESS;
 }
@@ -158,6 +196,10 @@ loops_status_t loops_destroy(loops_handle_t handle) {
   if (!handle) return LOOPS_ERROR_INVALID_VALUE;
   hipError_t e = hipSuccess;
   if (handle->ws) e = hipFree(handle->ws);
+  if (handle->counters) {
+    hipError_t e2 = hipFree(handle->counters);
+    if (e == hipSuccess) e = e2;
+  }
   delete handle;
   return hip_status(e);
 }
@@ -229,15 +271,21 @@ loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
   hipEvent_t ev0 = params ? params->start_event : nullptr;
   hipEvent_t ev1 = params ? params->stop_event : nullptr;
   switch (schedule) {
-    case LOOPS_SCHEDULE_MERGE_PATH_FLAT:
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT: {
+      int* counters = nullptr;
+      if (!(params && (params->flags & LOOPS_FLAG_SEPARATE_FIXUP))) {
+        counters = fixup_counters(handle, r.units, stream, &st);
+        if (st != LOOPS_SUCCESS) return st;
+      }
       return hip_status(launch_mp_spmv(a, x, y, r.tpb, r.ipt,
                                        params ? params->tile_coordinates : nullptr,
-                                       carve_fixup_ws(ws, r.units), ev0, ev1,
-                                       stream));
+                                       carve_fixup_ws(ws, r.units, counters),
+                                       counters != nullptr, ev0, ev1, stream));
+    }
     case LOOPS_SCHEDULE_WORK_ORIENTED:
       return hip_status(launch_wo_spmv(a, x, y, r.tpb, r.grid,
-                                       carve_fixup_ws(ws, r.units), ev0, ev1,
-                                       stream));
+                                       carve_fixup_ws(ws, r.units, nullptr), ev0,
+                                       ev1, stream));
     case LOOPS_SCHEDULE_GROUP_MAPPED:
       return hip_status(launch_gm_spmv(a, x, y, r.tpb, r.tpt, ev0, ev1, stream));
     case LOOPS_SCHEDULE_THREAD_MAPPED:

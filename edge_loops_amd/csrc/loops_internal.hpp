@@ -23,25 +23,30 @@ struct csr_args {
 };
 
 // Slots of the row fix-up: per work unit (merge tile or work_oriented
-// thread) the row continuing past its end and two partial sums.
+// thread) the row continuing past its end and two partial sums (caller
+// workspace, no initialisation needed).  counter: per unit, used only by the
+// in-kernel fix-up; it lives in memory owned by the handle that holds
+// nothing else, is zero before a call and is left zero by it (the last
+// participant of each row resets its counter).
 struct fixup_ws {
   int* carry_row;
   float* carry_val;
   float* head_val;
+  int* counter;
 };
 
 inline size_t fixup_ws_bytes(long long units) {
   size_t b = (size_t)units * (sizeof(int) + 2 * sizeof(float));
   return (b + 255) & ~size_t(255);
 }
-fixup_ws carve_fixup_ws(void* ws, long long units);
+fixup_ws carve_fixup_ws(void* ws, long long units, int* counters);
 
 // ---- SpMV ----------------------------------------------------------------
 bool mp_spmv_supported(int tpb, int ipt);
 hipError_t launch_mp_spmv(const csr_args& A, const float* x, float* y, int tpb,
                           int ipt, const loops_coord_t* coords,
-                          const fixup_ws& ws, hipEvent_t ev0, hipEvent_t ev1,
-                          hipStream_t s);
+                          const fixup_ws& ws, bool fused, hipEvent_t ev0,
+                          hipEvent_t ev1, hipStream_t s);
 hipError_t launch_mp_partition(const int* offsets, int rows, int nnz,
                                long long items_per_tile, long long num_coords,
                                loops_coord_t* coords, hipStream_t s);

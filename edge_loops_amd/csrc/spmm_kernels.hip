@@ -1,14 +1,26 @@
 // gfx950 SpMM kernels, C = A * B with B (cols x k) and C (rows x k) dense
 // row-major fp32 (leading dimensions ldb, ldc).
 //
-//   thread_mapped  algorithms/spmm/thread_mapped.cuh:28-53 (one row per
-//                  work-item).  The reference rescans the row once per output
+//   thread_mapped  algorithms/spmm/thread_mapped.cuh:28-53: one row per
+//                  work-item.  The reference rescans the row once per output
 //                  column; here a work-item keeps 8 columns in registers and
-//                  rescans once per 8 columns.  Every C(row, c) is still the
-//                  sequential CSR-order fp32 sum of the reference.
+//                  rescans once per 8 columns.  Each C(row, c) is still the
+//                  sequential CSR-order fp32 sum.
+//   group_mapped   (no reference SpMM): a group of G lanes owns G consecutive
+//                  rows and walks their nonzeros in CSR order; the group's
+//                  lanes span the output columns (CPL columns per lane, one
+//                  16-byte load of a B row slice per lane when CPL = 4), so
+//                  every B row is fetched as whole contiguous segments.
+//                  Bit-identical to the sequential CSR-order sum.
+//   merge_path     (no reference SpMM): the same lane-per-column groups, but
+//                  each group walks an equal share (IPU merge steps) of the
+//                  merge path of rows and nonzeros, so one 200k-nonzero row
+//                  is spread over many groups.  Rows crossing a share end are
+//                  finished by a fixed-order fix-up (as for SpMV), per column.
 #include <hip/hip_runtime.h>
 
 #include <loops/schedule.hxx>
+#include <loops/util/search.hxx>
 
 #include "device_common.hpp"
 #include "loops_internal.hpp"
@@ -17,6 +29,9 @@ namespace loops_mi355x {
 
 using loops::schedule::algorithms_t;
 
+// ---------------------------------------------------------------------------
+// thread_mapped
+// ---------------------------------------------------------------------------
 template <typename V>
 __global__ void spmm_thread_mapped_kernel(
     loops::schedule::setup<algorithms_t::thread_mapped, 1, 1, int, int,
@@ -46,6 +61,227 @@ __global__ void spmm_thread_mapped_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Column slice of one lane: CPL consecutive columns starting at c.
+// ---------------------------------------------------------------------------
+template <int CPL>
+struct cols_t {
+  float v[CPL];
+};
+
+template <int CPL>
+__device__ __forceinline__ cols_t<CPL> load_b(const float* __restrict__ B,
+                                              long long ldb, int col, int c,
+                                              int k) {
+  cols_t<CPL> r;
+  const float* p = B + (long long)col * ldb + c;
+  if constexpr (CPL == 4) {
+    if (c + 4 <= k) {
+      f32x4 q = *reinterpret_cast<const f32x4*>(p);
+      r.v[0] = q.x; r.v[1] = q.y; r.v[2] = q.z; r.v[3] = q.w;
+      return r;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) r.v[j] = (c + j < k) ? p[j] : 0.f;
+  return r;
+}
+
+template <int CPL>
+__device__ __forceinline__ void store_c(float* __restrict__ p, int c, int k,
+                                        const float* acc) {
+  if constexpr (CPL == 4) {
+    if (c + 4 <= k) {
+      f32x4 q;
+      q.x = acc[0]; q.y = acc[1]; q.z = acc[2]; q.w = acc[3];
+      *reinterpret_cast<f32x4*>(p) = q;
+      return;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CPL; ++j)
+    if (c + j < k) p[j] = acc[j];
+}
+
+// ---------------------------------------------------------------------------
+// group_mapped: group g owns rows [g*G, g*G + G); the group walks their
+// nonzeros in order, U at a time (U independent B-row loads in flight).
+// ---------------------------------------------------------------------------
+template <int TPB, int G, int CPL, typename V>
+__global__ __launch_bounds__(TPB) void spmm_group_mapped_kernel(
+    const int* __restrict__ offsets, const int* __restrict__ indices,
+    const V* __restrict__ values, int rows, int k, const float* __restrict__ B,
+    long long ldb, float* __restrict__ C, long long ldc) {
+  constexpr int U = 8;
+  const int lane = threadIdx.x & (G - 1);
+  const long long group = ((long long)blockIdx.x * TPB + threadIdx.x) / G;
+  const long long r0 = group * G;
+  if (r0 >= rows) return;  // uniform within the group
+  const long long r1 = r0 + G < rows ? r0 + G : rows;
+  for (int cb = 0; cb < k; cb += G * CPL) {
+    const int c = cb + lane * CPL;
+    for (long long row = r0; row < r1; ++row) {
+      float acc[CPL];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
+      const int e = offsets[row + 1];
+      for (int nz = offsets[row]; nz < e; nz += U) {
+        const int n = e - nz < U ? e - nz : U;
+        int cj[U];
+        float vj[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          cj[j] = j < n ? indices[nz + j] : 0;
+          vj[j] = j < n ? value_traits<V>::load1(values + nz + j) : 0.f;
+        }
+        cols_t<CPL> bj[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+          if (j < n) bj[j] = load_b<CPL>(B, ldb, cj[j], c, k);
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+          if (j < n) {
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
+          }
+      }
+      if (c < k) store_c<CPL>(C + row * ldc + c, c, k, acc);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// merge_path: group u walks merge diagonals [u*IPU, (u+1)*IPU).
+// Work space per group: carry_row (int), carry[k] and head[k] (fp32).
+// ---------------------------------------------------------------------------
+struct spmm_ws {
+  int* carry_row;
+  float* carry;  // [units][k]
+  float* head;   // [units][k]
+};
+
+inline size_t spmm_ws_bytes_impl(long long units, int k) {
+  size_t b = (size_t)units * sizeof(int);
+  b = (b + 255) & ~size_t(255);
+  b += 2 * (size_t)units * k * sizeof(float);
+  return (b + 255) & ~size_t(255);
+}
+
+inline spmm_ws carve_spmm_ws(void* ws, long long units, int k) {
+  char* p = static_cast<char*>(ws);
+  size_t b = ((size_t)units * sizeof(int) + 255) & ~size_t(255);
+  spmm_ws w;
+  w.carry_row = reinterpret_cast<int*>(p);
+  w.carry = reinterpret_cast<float*>(p + b);
+  w.head = w.carry + (size_t)units * k;
+  return w;
+}
+
+template <int TPB, int G, int CPL, typename V>
+__global__ __launch_bounds__(TPB) void spmm_merge_path_kernel(
+    const int* __restrict__ offsets, const int* __restrict__ indices,
+    const V* __restrict__ values, int rows, int nnz, int k,
+    const float* __restrict__ B, long long ldb, float* __restrict__ C,
+    long long ldc, long long ipu, long long units, spmm_ws ws) {
+  constexpr int U = 8;
+  const int lane = threadIdx.x & (G - 1);
+  const long long u = ((long long)blockIdx.x * TPB + threadIdx.x) / G;
+  if (u >= units) return;  // uniform within the group
+  const long long total = (long long)rows + nnz;
+  long long d0 = u * ipu, d1 = d0 + ipu;
+  if (d1 > total) d1 = total;
+  const auto st = loops::search::_binary_search<long long>(
+      d0, offsets + 1, 0LL, (long long)rows, (long long)nnz);
+  const auto en = loops::search::_binary_search<long long>(
+      d1, offsets + 1, 0LL, (long long)rows, (long long)nnz);
+  const int xs = (int)st.x, ys = (int)st.y, xe = (int)en.x, ye = (int)en.y;
+  const bool open_left = xs < xe && offsets[xs] < ys;
+
+  for (int cb = 0; cb < k; cb += G * CPL) {
+    const int c = cb + lane * CPL;
+    int x = xs, y = ys;
+    // Row ends of rows [rbase, rbase + G) held one per lane.
+    int rbase = xs;
+    int re_reg = offsets[(xs + 1 + lane) < rows ? (xs + 1 + lane) : rows];
+    int row_end = __shfl(re_reg, 0, G);
+    bool first = true;
+    float acc[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
+
+    auto emit = [&]() {
+      if (first && open_left) {
+        if (c < k) store_c<CPL>(ws.head + u * k + c, c, k, acc);
+      } else if (c < k) {
+        store_c<CPL>(C + (long long)x * ldc + c, c, k, acc);
+      }
+      first = false;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
+      ++x;
+      if (x - rbase >= G) {
+        rbase = x;
+        int r = x + 1 + lane;
+        re_reg = offsets[r < rows ? r : rows];
+      }
+      row_end = __shfl(re_reg, x - rbase, G);
+    };
+
+    while (y < ye) {
+      const int n = ye - y < U ? ye - y : U;
+      int cj[U];
+      float vj[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        cj[j] = j < n ? indices[y + j] : 0;
+        vj[j] = j < n ? value_traits<V>::load1(values + y + j) : 0.f;
+      }
+      cols_t<CPL> bj[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (j < n) bj[j] = load_b<CPL>(B, ldb, cj[j], c, k);
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (j < n) {
+          while (x < xe && row_end <= y) emit();
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
+          ++y;
+        }
+      }
+    }
+    while (x < xe) emit();
+    // acc now holds this group's part of row xe (if any).
+    if (c < k) store_c<CPL>(ws.carry + u * k + c, c, k, acc);
+  }
+  if (lane == 0)
+    ws.carry_row[u] = (xe < rows && offsets[xe] < ye) ? xe : -1;
+}
+
+// Per (unit, column): the unit where an open row starts sums the unit
+// partials of that row in unit order (see spmv_fixup_kernel).
+__global__ void spmm_fixup_kernel(const int* __restrict__ offsets, int rows,
+                                  int k, long long ipu, long long units,
+                                  spmm_ws ws, float* __restrict__ C,
+                                  long long ldc) {
+  const long long t = blockIdx.x;
+  if (t >= units) return;
+  const int r = ws.carry_row[t];
+  if (r < 0 || r >= rows) return;
+  const long long fr = (long long)offsets[r] + r;
+  if (fr / ipu != t) return;
+  const long long t_end = ((long long)offsets[r + 1] + r + 1 - 1) / ipu;
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    float s = ws.carry[t * k + c];
+    for (long long v = t + 1; v < t_end; ++v) s += ws.carry[v * k + c];
+    s += ws.head[t_end * k + c];
+    C[(long long)r * ldc + c] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
 namespace {
 inline hipError_t record(hipEvent_t e, hipStream_t s) {
   return e ? hipEventRecord(e, s) : hipSuccess;
@@ -69,6 +305,90 @@ hipError_t tm_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
   }
   return record(ev1, s);
 }
+
+// Lanes per group and columns per lane for a given k and B/C alignment.
+struct spmm_shape {
+  int g;
+  int cpl;
+};
+inline spmm_shape pick_shape(int k, bool vec4) {
+  if (vec4 && k >= 16) {
+    int g = 4;
+    while (g < 64 && g * 4 < k) g *= 2;
+    return {g, 4};
+  }
+  int g = 1;
+  while (g < 64 && g < k) g *= 2;
+  return {g, 1};
+}
+
+template <int TPB, int G, int CPL, typename V>
+hipError_t gm_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
+                     float* C, long long ldc, hipStream_t s) {
+  long long groups = ((long long)A.rows + G - 1) / G;
+  long long grid = (groups * G + TPB - 1) / TPB;
+  spmm_group_mapped_kernel<TPB, G, CPL, V><<<dim3((unsigned)grid), dim3(TPB), 0, s>>>(
+      A.offsets, A.indices, static_cast<const V*>(A.values), A.rows, k, B, ldb,
+      C, ldc);
+  return hipGetLastError();
+}
+
+template <int TPB, int G, int CPL, typename V>
+hipError_t mp_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
+                     float* C, long long ldc, long long ipu, void* ws,
+                     hipStream_t s, hipEvent_t ev1) {
+  const long long total = (long long)A.rows + A.nnz;
+  const long long units = (total + ipu - 1) / ipu;
+  spmm_ws w = carve_spmm_ws(ws, units, k);
+  long long grid = (units * G + TPB - 1) / TPB;
+  spmm_merge_path_kernel<TPB, G, CPL, V><<<dim3((unsigned)grid), dim3(TPB), 0, s>>>(
+      A.offsets, A.indices, static_cast<const V*>(A.values), A.rows, A.nnz, k,
+      B, ldb, C, ldc, ipu, units, w);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = record(ev1, s);
+  if (e != hipSuccess) return e;
+  if (units > 1) {
+    spmm_fixup_kernel<<<dim3((unsigned)units), dim3(64), 0, s>>>(
+        A.offsets, A.rows, k, ipu, units, w, C, ldc);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+#define LOOPS_SPMM_SHAPES(X) \
+  X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(64, 1) \
+  X(4, 4) X(8, 4) X(16, 4) X(32, 4) X(64, 4)
+
+template <typename V>
+hipError_t gm_spmm_v(const csr_args& A, int k, const float* B, long long ldb,
+                     float* C, long long ldc, spmm_shape sh, hipStream_t s) {
+#define X(G_, C_) \
+  if (sh.g == G_ && sh.cpl == C_) return gm_spmm_t<256, G_, C_, V>(A, k, B, ldb, C, ldc, s);
+  LOOPS_SPMM_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+template <typename V>
+hipError_t mp_spmm_v(const csr_args& A, int k, const float* B, long long ldb,
+                     float* C, long long ldc, spmm_shape sh, long long ipu,
+                     void* ws, hipStream_t s, hipEvent_t ev1) {
+#define X(G_, C_)                  \
+  if (sh.g == G_ && sh.cpl == C_) \
+    return mp_spmm_t<256, G_, C_, V>(A, k, B, ldb, C, ldc, ipu, ws, s, ev1);
+  LOOPS_SPMM_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+inline bool vec4_ok(const float* B, long long ldb, const float* C, long long ldc) {
+  return (reinterpret_cast<uintptr_t>(B) % 16 == 0) && (ldb % 4 == 0) &&
+         (reinterpret_cast<uintptr_t>(C) % 16 == 0) && (ldc % 4 == 0);
+}
+
+constexpr long long kSpmmDefaultIpu = 256;
+
 }  // namespace
 
 hipError_t launch_tm_spmm(const csr_args& A, int k, const float* B, long long ldb,
@@ -82,18 +402,55 @@ hipError_t launch_tm_spmm(const csr_args& A, int k, const float* B, long long ld
   return hipErrorInvalidValue;
 }
 
-bool gm_spmm_supported(int) { return false; }
-hipError_t launch_gm_spmm(const csr_args&, int, const float*, long long, float*,
-                          long long, int, hipEvent_t, hipEvent_t, hipStream_t) {
-  return hipErrorNotSupported;
+bool gm_spmm_supported(int tpb) { return tpb == 256; }
+
+hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ldb,
+                          float* C, long long ldc, int tpb, hipEvent_t ev0,
+                          hipEvent_t ev1, hipStream_t s) {
+  (void)tpb;
+  hipError_t e = record(ev0, s);
+  if (e != hipSuccess) return e;
+  if (A.rows > 0 && k > 0) {
+    spmm_shape sh = pick_shape(k, vec4_ok(B, ldb, C, ldc));
+    switch (A.vtype) {
+      case LOOPS_VALUE_F32: e = gm_spmm_v<float>(A, k, B, ldb, C, ldc, sh, s); break;
+      case LOOPS_VALUE_BF16: e = gm_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, s); break;
+      case LOOPS_VALUE_F16: e = gm_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, s); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  return record(ev1, s);
 }
-bool mp_spmm_supported(int, int) { return false; }
-long long mp_spmm_units(const csr_args&, int, int) { return 0; }
-size_t mp_spmm_ws_bytes(long long, int) { return 0; }
-hipError_t launch_mp_spmm(const csr_args&, int, const float*, long long, float*,
-                          long long, int, int, void*, hipEvent_t, hipEvent_t,
-                          hipStream_t) {
-  return hipErrorNotSupported;
+
+bool mp_spmm_supported(int tpb, int ipu) {
+  return tpb == 256 && (ipu == 0 || (ipu >= 16 && ipu <= 65536));
+}
+
+long long mp_spmm_units(const csr_args& A, int tpb, int ipu) {
+  (void)tpb;
+  long long per = ipu ? ipu : kSpmmDefaultIpu;
+  return ((long long)A.rows + A.nnz + per - 1) / per;
+}
+
+size_t mp_spmm_ws_bytes(long long units, int k) {
+  return units > 0 && k > 0 ? spmm_ws_bytes_impl(units, k) : 0;
+}
+
+hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ldb,
+                          float* C, long long ldc, int tpb, int ipu, void* ws,
+                          hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
+  (void)tpb;
+  hipError_t e = record(ev0, s);
+  if (e != hipSuccess) return e;
+  if ((long long)A.rows + A.nnz == 0 || k == 0 || A.rows == 0) return record(ev1, s);
+  const long long per = ipu ? ipu : kSpmmDefaultIpu;
+  spmm_shape sh = pick_shape(k, vec4_ok(B, ldb, C, ldc));
+  switch (A.vtype) {
+    case LOOPS_VALUE_F32: return mp_spmm_v<float>(A, k, B, ldb, C, ldc, sh, per, ws, s, ev1);
+    case LOOPS_VALUE_BF16: return mp_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, per, ws, s, ev1);
+    case LOOPS_VALUE_F16: return mp_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, per, ws, s, ev1);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace loops_mi355x

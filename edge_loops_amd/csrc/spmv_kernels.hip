@@ -10,11 +10,17 @@
 //
 // None of them uses floating-point atomics.  A row that crosses the end of a
 // work unit (a merge tile, or a work_oriented thread's share) is finished by
-// launch_fixup: each unit records the row it leaves open (carry_row), its
+// the fix-up: every unit records the row it leaves open (carry_row), its
 // partial sum of that row (carry_val) and its partial sum of the row it
-// entered already open (head_val).  The unit in which a row starts sums the
-// unit partials of that row in unit order, so every y[row] is produced by
-// one fixed sequence of fp32 additions on every run.
+// entered already open (head_val).  Row r occupies merge diagonals
+// [f(r), f(r+1)) with f(k) = offsets[k] + k, so the units holding it are
+// t0 = f(r) / U .. t1 = (f(r+1) - 1) / U (U = merge steps per unit); its
+// value is carry_val[t0] + ... + carry_val[t1 - 1] + head_val[t1], summed in
+// that order, so every y[row] comes from one fixed sequence of fp32
+// additions on every run.  The fix-up runs either as a second kernel, or
+// inside the merge-path kernel: each participating unit bumps a counter
+// (agent-scope acquire/release atomics) and the last of the n = t1 - t0 + 1
+// participants computes the sum and resets the counter to zero.
 #include <hip/hip_runtime.h>
 
 #include <loops/schedule.hxx>
@@ -27,46 +33,64 @@ namespace loops_mi355x {
 
 using loops::schedule::algorithms_t;
 
-fixup_ws carve_fixup_ws(void* ws, long long units) {
+fixup_ws carve_fixup_ws(void* ws, long long units, int* counters) {
   char* p = static_cast<char*>(ws);
   fixup_ws w;
   w.carry_row = reinterpret_cast<int*>(p);
   w.carry_val = reinterpret_cast<float*>(p + units * sizeof(int));
   w.head_val = reinterpret_cast<float*>(p + units * (sizeof(int) + sizeof(float)));
+  w.counter = counters;
   return w;
+}
+
+// Sum of carry[t0 .. t1-1] + head[t1] by one wavefront, in an order fixed by
+// (t1 - t0) alone: lane l adds elements l, l+64, ... in sequence, then the
+// 64 lane sums are combined by a fixed butterfly.  All 64 lanes must call.
+__device__ __forceinline__ float wave_segment_sum(const float* __restrict__ carry,
+                                                  const float* __restrict__ head,
+                                                  long long t0, long long t1) {
+  const int lane = threadIdx.x & 63;
+  float s = 0.f;
+  for (long long u = t0 + lane; u < t1; u += 64) s += carry[u];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  return s + head[t1];
 }
 
 // ---------------------------------------------------------------------------
 // Merge-path SpMV.
 //
 // One workgroup per merge tile of TI = TPB * IPT steps (rows + nonzeros).
-//  1. Tile end points: read from the partition, or found by two wavefronts,
-//     each running a 64-way cooperative search (loops::search::wave_search).
-//  2. The row ends of the tile's rows go to LDS; the tile's nonzeros are
-//     streamed with 16-byte non-temporal loads (values and column ids) and
-//     their products a_ij * x_j go to LDS, so every HBM byte of A is read
-//     once, coalesced, and x keeps the L2.
-//  3. Each lane finds its start with a search in LDS and walks IPT merge
-//     steps (consume a product or close a row), writing closed rows to LDS.
-//  4. A workgroup segmented scan carries each lane's trailing partial into
-//     the next lane's first closed row.
-//  5. Closed rows are stored coalesced; the open ends go to the fix-up slots.
+//  1. Tile end points: read from a precomputed partition, or found by two
+//     wavefronts, each running a 64-way cooperative search
+//     (loops::search::wave_search).
+//  2. The row ends of the tile's rows go to LDS.  The tile's nonzeros are
+//     streamed with 16-byte non-temporal loads (column ids and values, all
+//     issued before any x gather), and the products a_ij * x_j go to LDS, so
+//     A is read once, coalesced, and x keeps the L2.
+//  3. Each lane finds its start with a binary search in LDS and walks IPT
+//     merge steps (consume a product, or close a row and store y).  A lane
+//     keeps its first closed row back.
+//  4. A workgroup segmented scan gives each lane the partial sum carried in
+//     from earlier lanes; it is added to the lane's first closed row.
+//  5. The row entering the tile already open and the row leaving it open go
+//     through the fix-up (slots + optional in-kernel last-arriver).
 // ---------------------------------------------------------------------------
-template <int TPB, int IPT, typename V, bool VEC>
+template <int TPB, int IPT, typename V, bool VEC, bool FUSED>
 __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
     const int* __restrict__ offsets, const int* __restrict__ indices,
     const V* __restrict__ values, const float* __restrict__ x,
     float* __restrict__ y, int rows, int nnz,
-    const loops_coord_t* __restrict__ coords, int* __restrict__ carry_row,
-    float* __restrict__ carry_val, float* __restrict__ head_val) {
+    const loops_coord_t* __restrict__ coords, fixup_ws ws) {
   constexpr int TI = TPB * IPT;
   constexpr int NW = TPB / 64;
+  constexpr int NC = (TI + 4 * TPB - 1) / (4 * TPB);  // 4-wide chunks per lane
   __shared__ int s_rowend[TI + 1];
   __shared__ __align__(16) float s_prod[TI + 4];
-  __shared__ float s_y[TI];
   __shared__ int s_coord[5];
   __shared__ float s_wv[NW];
   __shared__ int s_wf[NW];
+  __shared__ float s_part[2];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -116,33 +140,52 @@ __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
   }
   if (tid == 0) s_coord[4] = offsets[xs] - ys;
 
-  // 2b. products of the tile's nonzeros [ys, ye)
+  // 2b. products of the tile's nonzeros [ys, ye): every column id and value
+  // load first, then the x gathers, then the LDS stores.
   const int ybase = ys & ~3;
   const int pofs = ys - ybase;
-  if (VEC) {
-    for (int j = ybase + 4 * tid; j < ye; j += 4 * TPB) {
-      if (j >= ys && j + 4 <= ye) {
-        i32x4 c = ld_idx4(indices + j);
-        f32x4 v = value_traits<V>::load4(values + j);
-        f32x4 p;
-        p.x = v.x * x[c.x];
-        p.y = v.y * x[c.y];
-        p.z = v.z * x[c.z];
-        p.w = v.w * x[c.w];
-        *reinterpret_cast<f32x4*>(&s_prod[j - ybase]) = p;
+  {
+    i32x4 ci[NC];
+    f32x4 cv[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int j = ybase + 4 * (tid + q * TPB);
+      if (VEC && j >= ys && j + 4 <= ye) {
+        ci[q] = ld_idx4(indices + j);
+        cv[q] = value_traits<V>::load4(values + j);
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          int jj = j + e;
-          if (jj >= ys && jj < ye)
-            s_prod[jj - ybase] =
-                value_traits<V>::load1(values + jj) * x[ld_idx(indices + jj)];
+          const int jj = j + e;
+          const bool ok = jj >= ys && jj < ye;
+          ci[q][e] = ok ? ld_idx(indices + jj) : 0;
+          cv[q][e] = ok ? value_traits<V>::load1(values + jj) : 0.f;
         }
       }
     }
-  } else {
-    for (int j = ys + tid; j < ye; j += TPB)
-      s_prod[j - ybase] = value_traits<V>::load1(values + j) * x[ld_idx(indices + j)];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int j = ybase + 4 * (tid + q * TPB);
+      if (j < ye) {
+        f32x4 p;
+        p.x = cv[q].x * x[ci[q].x];
+        p.y = cv[q].y * x[ci[q].y];
+        p.z = cv[q].z * x[ci[q].z];
+        p.w = cv[q].w * x[ci[q].w];
+        if (j >= ys && j + 4 <= ye) {
+          *reinterpret_cast<f32x4*>(&s_prod[j - ybase]) = p;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (j + e >= ys && j + e < ye) s_prod[j + e - ybase] = p[e];
+        }
+      }
+    }
+    // The chunks cover [ybase, ybase + 4 * TPB * NC), at least TI entries;
+    // a tile of TI nonzeros starting off a 4-boundary ends up to 3 beyond.
+    const int jt = ybase + 4 * TPB * NC + tid;
+    if (jt < ye)
+      s_prod[jt - ybase] = value_traits<V>::load1(values + jt) * x[ld_idx(indices + jt)];
   }
   __syncthreads();
 
@@ -171,7 +214,8 @@ __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
     yl = dd - first;
   }
   float run = 0.f;
-  int first_emit = -1;
+  int first_row = -1;
+  float first_val = 0.f;
 #pragma unroll
   for (int s = 0; s < IPT; ++s) {
     if (s < steps) {
@@ -179,17 +223,21 @@ __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
         run += s_prod[yl + pofs];
         ++yl;
       } else {
-        s_y[xl] = run;
-        if (first_emit < 0) first_emit = xl;
+        if (first_row < 0) {
+          first_row = xl;
+          first_val = run;
+        } else {
+          y[xs + xl] = run;
+        }
         run = 0.f;
         ++xl;
       }
     }
   }
 
-  // 4. carry each lane's trailing partial into the next closed row
+  // 4. partial sum carried in from earlier lanes
   float v = run;
-  int f = first_emit >= 0 ? 1 : 0;
+  int f = first_row >= 0 ? 1 : 0;
   group_seg_scan<64>(v, f);
   if (lane == 63) {
     s_wv[wave] = v;
@@ -203,59 +251,123 @@ __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
   }
   float ev = __shfl_up(v, 1, 64);
   int ef = __shfl_up(f, 1, 64);
-  float excl = lane == 0 ? pv : (ef ? ev : pv + ev);
-  if (first_emit >= 0) s_y[first_emit] = excl + s_y[first_emit];
-  __syncthreads();
-
-  // 5. store closed rows; record the open ends
+  const float excl = lane == 0 ? pv : (ef ? ev : pv + ev);
   const bool open_left = tile_rows > 0 && s_coord[4] < 0;
-  for (int i = tid; i < tile_rows; i += TPB) {
-    if (i == 0 && open_left) continue;
-    y[xs + i] = s_y[i];
+  if (first_row >= 0) {
+    const float val = excl + first_val;
+    if (first_row == 0 && open_left) {
+      if (FUSED) s_part[0] = val;
+      else ws.head_val[tile] = val;
+    } else {
+      y[xs + first_row] = val;
+    }
   }
+
+  // 5. the row leaving the tile open
+  const int xe_start = tile_rows > 0 ? s_rowend[tile_rows - 1] : s_coord[4];
+  const bool open_right = xe < rows && xe_start < tile_nnz;
   if (tid == TPB - 1) {
-    float tot = f ? v : pv + v;
-    const int xe_start = tile_rows > 0 ? s_rowend[tile_rows - 1] : s_coord[4];
-    carry_row[tile] = (xe < rows && xe_start < tile_nnz) ? xe : -1;
-    carry_val[tile] = tot;
-    head_val[tile] = open_left ? s_y[0] : 0.f;
+    const float tot = f ? v : pv + v;
+    if (FUSED) {
+      s_part[1] = tot;
+    } else {
+      ws.carry_row[tile] = open_right ? xe : -1;
+      ws.carry_val[tile] = tot;
+    }
+  }
+  if (!FUSED) return;
+
+  // In-kernel fix-up: wave 0 takes the open-left row, the last wave the
+  // open-right row.  Each bumps the row's counter; the last participant sums
+  // the slots of all participants in unit order.
+  __syncthreads();
+  const int part_wave_left = 0, part_wave_right = NW - 1;
+  for (int which = 0; which < 2; ++which) {
+    const bool mine = which == 0 ? (wave == part_wave_left && open_left)
+                                 : (wave == part_wave_right && open_right);
+    if (!mine) continue;
+    const long long r = which == 0 ? xs : xe;
+    const long long fr = which == 0 ? (long long)s_coord[4] + ys + xs
+                                    : (long long)xe_start + ys + xe;
+    const long long fr1 = which == 0 ? (long long)s_rowend[0] + ys + xs + 1
+                                     : (long long)s_rowend[tile_rows] + ys + xe + 1;
+    const long long t0 = fr / TI;
+    const long long t1 = (fr1 - 1) / TI;
+    const int n = (int)(t1 - t0 + 1);
+    int last = 0;
+    if (lane == 0) {
+      // Publish this tile's partial, then count in (release + acquire).
+      if (which == 0) ws.head_val[tile] = s_part[0];
+      else ws.carry_val[tile] = s_part[1];
+      int old = __hip_atomic_fetch_add(&ws.counter[t0], 1, __ATOMIC_ACQ_REL,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+      last = old == n - 1;
+    }
+    last = __shfl(last, 0, 64);
+    if (last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      const float s = wave_segment_sum(ws.carry_val, ws.head_val, t0, t1);
+      if (lane == 0) {
+        y[r] = s;
+        __hip_atomic_store(&ws.counter[t0], 0, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 
-// Finishes rows that cross work-unit ends (see the file comment).  Unit u
-// covers merge diagonals [u * unit_items, (u + 1) * unit_items).
-__global__ void spmv_fixup_kernel(const int* __restrict__ offsets, int rows,
-                                  long long unit_items, long long num_units,
-                                  const int* __restrict__ carry_row,
-                                  const float* __restrict__ carry_val,
-                                  const float* __restrict__ head_val,
-                                  float* __restrict__ y) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= num_units) return;
-  int r = carry_row[t];
-  if (r < 0 || r >= rows) return;
-  long long fr = (long long)offsets[r] + r;
-  if (fr / unit_items != t) return;  // not the unit where row r starts
-  long long fr1 = (long long)offsets[r + 1] + r + 1;
-  long long t_end = (fr1 - 1) / unit_items;
-  float s = carry_val[t];
-  for (long long u = t + 1; u < t_end; ++u) s += carry_val[u];
-  s += head_val[t_end];
-  y[r] = s;
+// Second-kernel fix-up (see the file comment).  One wavefront per 64 units;
+// the unit where an open row starts owns its sum, computed cooperatively by
+// the whole wavefront with the same fixed order as the in-kernel path.
+__global__ __launch_bounds__(256) void spmv_fixup_kernel(
+    const int* __restrict__ offsets, int rows, long long unit_items,
+    long long num_units, fixup_ws ws, float* __restrict__ y) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  long long t0 = -1, t1 = -1;
+  int r = -1;
+  if (t < num_units) {
+    r = ws.carry_row[t];
+    if (r >= 0 && r < rows) {
+      const long long fr = (long long)offsets[r] + r;
+      if (fr / unit_items == t) {
+        t0 = t;
+        t1 = ((long long)offsets[r + 1] + r + 1 - 1) / unit_items;
+      }
+    }
+  }
+  unsigned long long todo = __ballot(t0 >= 0);
+  while (todo) {
+    const int src = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    const long long a = __shfl(t0, src, 64);
+    const long long b = __shfl(t1, src, 64);
+    const int rr = __shfl(r, src, 64);
+    const float s = wave_segment_sum(ws.carry_val, ws.head_val, a, b);
+    if (lane == src) y[rr] = s;
+  }
 }
 
-// One wavefront per coordinate: coords[c] = MP(min(c * items_per_tile,
-// rows + nnz)).
+// Merge-path partition by scatter: work-item r writes the coordinate of
+// every tile boundary t * items_per_tile that falls in row r's diagonals
+// [f(r), f(r+1)), which is (r, t * items_per_tile - r); one extra work-item
+// writes the end of the path (rows, nnz) at t = num_coords - 1.  Reads the
+// offsets once, in order; equal to the search at every boundary because
+// MP(d).x = max{k : f(k) <= d}.
 __global__ __launch_bounds__(256) void mp_partition_kernel(
     const int* __restrict__ offsets, int rows, int nnz, long long items_per_tile,
     long long num_coords, loops_coord_t* __restrict__ coords) {
-  long long c = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (c >= num_coords) return;  // wave-uniform
-  long long total = (long long)rows + nnz;
-  long long d = c * items_per_tile;
-  if (d > total) d = total;
-  auto xy = loops::search::wave_search(offsets, rows, nnz, d);
-  if ((threadIdx.x & 63) == 0) coords[c] = loops_coord_t{xy.x, xy.y};
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > rows) return;
+  if (r == rows) {
+    coords[num_coords - 1] = loops_coord_t{(unsigned)rows, (unsigned)nnz};
+    return;
+  }
+  const long long a = (long long)offsets[r] + r;
+  const long long b = (long long)offsets[r + 1] + r + 1;
+  for (long long t = (a + items_per_tile - 1) / items_per_tile;
+       t * items_per_tile < b && t < num_coords - 1; ++t)
+    coords[t] = loops_coord_t{(unsigned)r, (unsigned)(t * items_per_tile - r)};
 }
 
 // work_oriented partition (work_oriented.hxx:150-167) for num_threads
@@ -429,7 +541,7 @@ inline hipError_t record(hipEvent_t e, hipStream_t s) {
   return e ? hipEventRecord(e, s) : hipSuccess;
 }
 
-template <int TPB, int IPT, typename V>
+template <int TPB, int IPT, typename V, bool FUSED>
 hipError_t mp_launch_t(const csr_args& A, const float* x, float* y,
                        const loops_coord_t* coords, const fixup_ws& ws,
                        hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
@@ -441,33 +553,35 @@ hipError_t mp_launch_t(const csr_args& A, const float* x, float* y,
   if (tiles > 0) {
     const V* vals = static_cast<const V*>(A.values);
     if (A.aligned)
-      spmv_merge_path_kernel<TPB, IPT, V, true><<<dim3((unsigned)tiles), dim3(TPB), 0, s>>>(
-          A.offsets, A.indices, vals, x, y, A.rows, A.nnz, coords, ws.carry_row,
-          ws.carry_val, ws.head_val);
+      spmv_merge_path_kernel<TPB, IPT, V, true, FUSED><<<dim3((unsigned)tiles), dim3(TPB), 0, s>>>(
+          A.offsets, A.indices, vals, x, y, A.rows, A.nnz, coords, ws);
     else
-      spmv_merge_path_kernel<TPB, IPT, V, false><<<dim3((unsigned)tiles), dim3(TPB), 0, s>>>(
-          A.offsets, A.indices, vals, x, y, A.rows, A.nnz, coords, ws.carry_row,
-          ws.carry_val, ws.head_val);
+      spmv_merge_path_kernel<TPB, IPT, V, false, FUSED><<<dim3((unsigned)tiles), dim3(TPB), 0, s>>>(
+          A.offsets, A.indices, vals, x, y, A.rows, A.nnz, coords, ws);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   e = record(ev1, s);
   if (e != hipSuccess) return e;
-  if (tiles > 1) return launch_fixup(A.offsets, A.rows, TI, tiles, ws, y, s);
+  if (!FUSED && tiles > 1) return launch_fixup(A.offsets, A.rows, TI, tiles, ws, y, s);
   return hipSuccess;
 }
 
 template <int TPB, int IPT>
 hipError_t mp_dispatch_v(const csr_args& A, const float* x, float* y,
                          const loops_coord_t* coords, const fixup_ws& ws,
-                         hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
+                         bool fused, hipEvent_t ev0, hipEvent_t ev1,
+                         hipStream_t s) {
   switch (A.vtype) {
     case LOOPS_VALUE_F32:
-      return mp_launch_t<TPB, IPT, float>(A, x, y, coords, ws, ev0, ev1, s);
+      return fused ? mp_launch_t<TPB, IPT, float, true>(A, x, y, coords, ws, ev0, ev1, s)
+                   : mp_launch_t<TPB, IPT, float, false>(A, x, y, coords, ws, ev0, ev1, s);
     case LOOPS_VALUE_BF16:
-      return mp_launch_t<TPB, IPT, bf16_t>(A, x, y, coords, ws, ev0, ev1, s);
+      return fused ? mp_launch_t<TPB, IPT, bf16_t, true>(A, x, y, coords, ws, ev0, ev1, s)
+                   : mp_launch_t<TPB, IPT, bf16_t, false>(A, x, y, coords, ws, ev0, ev1, s);
     case LOOPS_VALUE_F16:
-      return mp_launch_t<TPB, IPT, f16_t>(A, x, y, coords, ws, ev0, ev1, s);
+      return fused ? mp_launch_t<TPB, IPT, f16_t, true>(A, x, y, coords, ws, ev0, ev1, s)
+                   : mp_launch_t<TPB, IPT, f16_t, false>(A, x, y, coords, ws, ev0, ev1, s);
   }
   return hipErrorInvalidValue;
 }
@@ -507,11 +621,11 @@ bool mp_spmv_supported(int tpb, int ipt) {
 
 hipError_t launch_mp_spmv(const csr_args& A, const float* x, float* y, int tpb,
                           int ipt, const loops_coord_t* coords,
-                          const fixup_ws& ws, hipEvent_t ev0, hipEvent_t ev1,
-                          hipStream_t s) {
+                          const fixup_ws& ws, bool fused, hipEvent_t ev0,
+                          hipEvent_t ev1, hipStream_t s) {
 #define X(T, I)          \
   if (tpb == T && ipt == I) \
-    return mp_dispatch_v<T, I>(A, x, y, coords, ws, ev0, ev1, s);
+    return mp_dispatch_v<T, I>(A, x, y, coords, ws, fused, ev0, ev1, s);
   LOOPS_MP_TUNINGS(X)
 #undef X
   return hipErrorInvalidValue;
@@ -524,8 +638,7 @@ hipError_t launch_fixup(const int* offsets, int rows, long long unit_items,
   const int tpb = 256;
   long long grid = (num_units + tpb - 1) / tpb;
   spmv_fixup_kernel<<<dim3((unsigned)grid), dim3(tpb), 0, s>>>(
-      offsets, rows, unit_items, num_units, ws.carry_row, ws.carry_val,
-      ws.head_val, y);
+      offsets, rows, unit_items, num_units, ws, y);
   return hipGetLastError();
 }
 
@@ -533,7 +646,7 @@ hipError_t launch_mp_partition(const int* offsets, int rows, int nnz,
                                long long items_per_tile, long long num_coords,
                                loops_coord_t* coords, hipStream_t s) {
   if (num_coords <= 0) return hipSuccess;
-  long long grid = (num_coords + 3) / 4;  // 4 wavefronts per 256-lane block
+  long long grid = ((long long)rows + 1 + 255) / 256;  // one work-item per row, +1
   mp_partition_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(
       offsets, rows, nnz, items_per_tile, num_coords, coords);
   return hipGetLastError();

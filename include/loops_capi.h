@@ -96,6 +96,11 @@ typedef struct loops_coord_t {
  *   work_oriented:   128 threads, grid 2 x CU count (work_oriented.cuh:97-104)
  *   group_mapped:    block_mapped<128> (group_mapped.cuh:39,79)
  *   thread_mapped / original: 128 threads, ceil(rows/128) blocks. */
+/* loops_params_t.flags */
+#define LOOPS_FLAG_SEPARATE_FIXUP 1u /* merge_path_flat: finish rows that
+                                        cross tiles in a second kernel
+                                        instead of inside the main kernel */
+
 typedef struct loops_params_t {
   int32_t threads_per_block;
   int32_t items_per_thread;  /* merge_path_flat only                       */
@@ -109,6 +114,7 @@ typedef struct loops_params_t {
    * util::timer_t returned by merge_path_flat (merge_path_flat.cuh:121-138). */
   hipEvent_t start_event;
   hipEvent_t stop_event;
+  uint32_t flags;
 } loops_params_t;
 
 typedef struct loops_handle* loops_handle_t;
@@ -135,7 +141,12 @@ loops_status_t loops_spmv_workspace_size(loops_handle_t handle,
 
 /* y = A * x (y overwritten).  workspace may be NULL, in which case the
  * handle's own workspace is used (grown with hipMalloc outside any graph
- * capture).  Asynchronous on stream. */
+ * capture); a caller workspace needs no initialisation.  merge_path_flat
+ * finishes rows that cross tiles inside its kernel with counters owned by
+ * the handle (grown outside graph capture; inside a capture that needs more
+ * than the handle holds, the separate fix-up kernel is used).  Calls that
+ * overlap in time must use different handles and workspaces.  Asynchronous
+ * on stream. */
 loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
                           const loops_csr_view_t* A, const float* x, float* y,
                           const loops_params_t* params, void* workspace,

@@ -1,0 +1,24 @@
+/**
+ * @file group_mapped.cuh
+ * @brief SpMM C = A B (B: cols x n, C: rows x n, dense row-major fp32,
+ * container/matrix.cuh), group_mapped schedule.
+ * No counterpart in the reference: a group of lanes owns consecutive rows and its lanes span the columns of B and C.  Blocks until done.
+ */
+#pragma once
+
+#include <loops/algorithms/detail/capi.hxx>
+#include <loops/container/matrix.cuh>
+
+namespace loops {
+namespace algorithms {
+namespace spmm {
+
+template <typename index_t, typename offset_t, typename type_t>
+void group_mapped(csr_t<index_t, offset_t, type_t>& csr, matrix_t<type_t>& B,
+          matrix_t<type_t>& C, hipStream_t stream = 0) {
+  detail::spmm(LOOPS_SCHEDULE_GROUP_MAPPED, csr, B, C, stream);
+}
+
+}  // namespace spmm
+}  // namespace algorithms
+}  // namespace loops

@@ -12,6 +12,7 @@
  */
 #pragma once
 
+#include <cctype>
 #include <cstdio>
 #include <cstring>
 #include <limits>

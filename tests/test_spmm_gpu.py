@@ -1,0 +1,104 @@
+"""GPU parity of SpMM C = A B through the C ABI.
+
+thread_mapped and group_mapped sum every C(row, c) in CSR order, so they
+must equal the fp32 oracle (oracle_spmm_f32, the column-wise restatement of
+algorithms/spmm/thread_mapped.cuh:41-52) bit for bit; merge_path splits long
+rows and is held to |C - C64| <= 1e-5 * sum_j |a_ij b_jc| per entry against
+the FP64 oracle.  SpMM parity is pinned by the oracle's closed form
+(column-wise SpMV); the reference has no SpMM validator.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import random_csr
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import edge_loops_amd as loops  # noqa: E402
+from edge_loops_amd import synth  # noqa: E402
+
+REL = 1e-5
+
+
+def _mats():
+    m = {}
+    m["sample4"] = (np.array([0, 0, 2, 3, 4], np.int32), np.array([0, 1, 2, 1], np.int32),
+                    np.array([5, 8, 3, 6], np.float32), 4)
+    off, idx, val = random_csr(3000, 2500, 40, seed=1, long_rows=((7, 2500), (2999, 1700)))
+    m["skewed"] = (off, idx, val, 2500)
+    lens = np.array([3, 9001, 1, 7003, 2, 0, 4099, 5], np.int64)
+    off = np.zeros(lens.shape[0] + 1, np.int32)
+    np.cumsum(lens, out=off[1:])
+    rng = np.random.default_rng(4)
+    idx = np.concatenate([np.sort(rng.choice(12000, L, replace=False)) for L in lens]).astype(np.int32)
+    m["long_misaligned"] = (off, idx, rng.uniform(-1, 1, idx.shape[0]).astype(np.float32), 12000)
+    B = synth.powerlaw_csr(20000, 30000, 2.0, 1, 8000, 200000, seed=9, force_first_max=True)
+    m["powerlaw_20k"] = (B.offsets, B.indices, B.values, B.cols)
+    m["no_nnz"] = (np.zeros(65, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), 5)
+    return m
+
+
+MATS = _mats()
+
+
+def _run(schedule, name, k, ld_pad=0, value_type="f32", **tuning):
+    off, idx, val, cols = MATS[name]
+    rows = off.shape[0] - 1
+    A = loops.csr_t.from_host(rows, cols, off, idx, val, value_type=value_type)
+    Bh = synth.dense(cols, k, seed=5 + k)
+    Bfull = torch.zeros((cols, k + ld_pad), device="cuda")
+    Bfull[:, :k] = torch.from_numpy(Bh).cuda()
+    Cfull = torch.full((rows, k + ld_pad), float("nan"), device="cuda")
+    B, C = Bfull[:, :k], Cfull[:, :k]
+    loops.run_spmm(schedule, A, B, C, **tuning)
+    torch.cuda.synchronize()
+    if ld_pad:
+        assert torch.isnan(Cfull[:, k:]).all()  # padding untouched
+    return C.cpu().numpy(), Bh, A
+
+
+@pytest.mark.parametrize("schedule", ["thread_mapped", "group_mapped", "merge_path"])
+@pytest.mark.parametrize("name", list(MATS))
+@pytest.mark.parametrize("k", [1, 3, 10, 16, 64, 100])
+def test_spmm_parity(schedule, name, k):
+    off, idx, val, cols = MATS[name]
+    C, Bh, _ = _run(schedule, name, k)
+    C32 = oracle.spmm_f32(off, idx, val, Bh)
+    C64, A64 = oracle.spmm_f64(off, idx, val, Bh)
+    assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30), (schedule, name, k)
+    if schedule != "merge_path":
+        assert np.array_equal(C, C32), (schedule, name, k)
+
+
+@pytest.mark.parametrize("schedule", ["thread_mapped", "group_mapped", "merge_path"])
+@pytest.mark.parametrize("k,pad", [(8, 1), (32, 3), (64, 4), (130, 2)])
+def test_spmm_leading_dimension(schedule, k, pad):
+    off, idx, val, cols = MATS["skewed"]
+    C, Bh, _ = _run(schedule, "skewed", k, ld_pad=pad)
+    C64, A64 = oracle.spmm_f64(off, idx, val, Bh)
+    assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30)
+
+
+@pytest.mark.parametrize("ipu", [16, 64, 300, 1024, 4096])
+def test_spmm_merge_path_unit_sizes(ipu):
+    off, idx, val, cols = MATS["powerlaw_20k"]
+    C, Bh, _ = _run("merge_path", "powerlaw_20k", 64, items_per_thread=ipu)
+    C64, A64 = oracle.spmm_f64(off, idx, val, Bh)
+    assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30)
+
+
+def test_spmm_deterministic():
+    runs = [_run("merge_path", "powerlaw_20k", 32)[0] for _ in range(3)]
+    assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[1], runs[2])
+
+
+@pytest.mark.parametrize("vt", ["bf16", "f16"])
+def test_spmm_low_precision_storage(vt):
+    off, idx, val, cols = MATS["powerlaw_20k"]
+    for s in ("thread_mapped", "group_mapped", "merge_path"):
+        C, Bh, A = _run(s, "powerlaw_20k", 16, value_type=vt)
+        stored = A.values.float().cpu().numpy()
+        C64, A64 = oracle.spmm_f64(off, idx, stored, Bh)
+        assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30), s

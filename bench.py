@@ -47,7 +47,24 @@ def parse():
                     help="comma list of extra schedules to time, 'all' or 'none'")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="skip extras (profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes (--steps times); "
+                         "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     return ap.parse_args()
+
+
+def copy_calibration(mb, steps):
+    """A plain copy kernel of known size: reads and writes mb * 2^20 bytes
+    per launch, the reference point for the PMC byte counters."""
+    import torch
+    n = mb * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": n * 4, "launches": steps}), flush=True)
 
 
 def traffic_from_profile():
@@ -62,6 +79,9 @@ def traffic_from_profile():
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, args.steps)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build "
               "does not run multi-GPU jobs.", file=sys.stderr)

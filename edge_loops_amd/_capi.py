@@ -76,7 +76,7 @@ class Params(ctypes.Structure):
     ]
 
 
-FLAG_SEPARATE_FIXUP = 1
+FLAG_INKERNEL_FIXUP = 1
 
 
 class LoopsError(RuntimeError):

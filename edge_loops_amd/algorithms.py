@@ -34,14 +34,14 @@ def _ptr(t):
 
 def make_params(threads_per_block=0, items_per_thread=0, threads_per_tile=0,
                 grid_size=0, tile_coordinates=None, start_event=None,
-                stop_event=None, separate_fixup=False):
+                stop_event=None, inkernel_fixup=False):
     return _capi.Params(
         int(threads_per_block), int(items_per_thread), int(threads_per_tile),
         int(grid_size),
         _ptr(tile_coordinates) if tile_coordinates is not None else None,
         start_event.ptr if start_event is not None else None,
         stop_event.ptr if stop_event is not None else None,
-        _capi.FLAG_SEPARATE_FIXUP if separate_fixup else 0)
+        _capi.FLAG_INKERNEL_FIXUP if inkernel_fixup else 0)
 
 
 def workspace(schedule, csr, stream=None, **tuning):

@@ -273,7 +273,7 @@ loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
   switch (schedule) {
     case LOOPS_SCHEDULE_MERGE_PATH_FLAT: {
       int* counters = nullptr;
-      if (!(params && (params->flags & LOOPS_FLAG_SEPARATE_FIXUP))) {
+      if (params && (params->flags & LOOPS_FLAG_INKERNEL_FIXUP)) {
         counters = fixup_counters(handle, r.units, stream, &st);
         if (st != LOOPS_SUCCESS) return st;
       }

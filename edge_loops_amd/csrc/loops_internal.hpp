@@ -28,15 +28,18 @@ struct csr_args {
 // in-kernel fix-up; it lives in memory owned by the handle that holds
 // nothing else, is zero before a call and is left zero by it (the last
 // participant of each row resets its counter).
+// seg_end[u]: for the unit where its open row starts, the unit holding that
+// row's end (so the fix-up needs no offsets lookup); -1 otherwise.
 struct fixup_ws {
   int* carry_row;
+  int* seg_end;
   float* carry_val;
   float* head_val;
   int* counter;
 };
 
 inline size_t fixup_ws_bytes(long long units) {
-  size_t b = (size_t)units * (sizeof(int) + 2 * sizeof(float));
+  size_t b = (size_t)units * (2 * sizeof(int) + 2 * sizeof(float));
   return (b + 255) & ~size_t(255);
 }
 fixup_ws carve_fixup_ws(void* ws, long long units, int* counters);

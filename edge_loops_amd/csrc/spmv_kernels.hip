@@ -37,8 +37,9 @@ fixup_ws carve_fixup_ws(void* ws, long long units, int* counters) {
   char* p = static_cast<char*>(ws);
   fixup_ws w;
   w.carry_row = reinterpret_cast<int*>(p);
-  w.carry_val = reinterpret_cast<float*>(p + units * sizeof(int));
-  w.head_val = reinterpret_cast<float*>(p + units * (sizeof(int) + sizeof(float)));
+  w.seg_end = reinterpret_cast<int*>(p + units * sizeof(int));
+  w.carry_val = reinterpret_cast<float*>(p + units * 2 * sizeof(int));
+  w.head_val = reinterpret_cast<float*>(p + units * (2 * sizeof(int) + sizeof(float)));
   w.counter = counters;
   return w;
 }
@@ -55,6 +56,19 @@ __device__ __forceinline__ float wave_segment_sum(const float* __restrict__ carr
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
   return s + head[t1];
+}
+
+// The value of an open row: one work-item for a two-unit row (its one carry
+// plus the head), the whole wavefront otherwise.  Both paths give the same
+// bits for a two-unit row (the butterfly only adds exact zeros to it), so a
+// row's value never depends on which path computed it.
+__device__ __forceinline__ bool segment_is_short(long long t0, long long t1) {
+  return t1 - t0 <= 1;
+}
+__device__ __forceinline__ float short_segment_sum(const float* __restrict__ carry,
+                                                   const float* __restrict__ head,
+                                                   long long t0, long long t1) {
+  return (t0 < t1 ? carry[t0] : 0.f) + head[t1];
 }
 
 // ---------------------------------------------------------------------------
@@ -273,6 +287,13 @@ __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
     } else {
       ws.carry_row[tile] = open_right ? xe : -1;
       ws.carry_val[tile] = tot;
+      int seg_end = -1;
+      if (open_right) {
+        const long long fr = (long long)xe_start + ys + xe;
+        if (fr / TI == tile)
+          seg_end = (int)(((long long)s_rowend[tile_rows] + ys + xe + 1 - 1) / TI);
+      }
+      ws.seg_end[tile] = seg_end;
     }
   }
   if (!FUSED) return;
@@ -306,7 +327,9 @@ __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
     last = __shfl(last, 0, 64);
     if (last) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      const float s = wave_segment_sum(ws.carry_val, ws.head_val, t0, t1);
+      const float s = segment_is_short(t0, t1)
+                          ? short_segment_sum(ws.carry_val, ws.head_val, t0, t1)
+                          : wave_segment_sum(ws.carry_val, ws.head_val, t0, t1);
       if (lane == 0) {
         y[r] = s;
         __hip_atomic_store(&ws.counter[t0], 0, __ATOMIC_RELAXED,
@@ -316,35 +339,32 @@ __global__ __launch_bounds__(TPB) void spmv_merge_path_kernel(
   }
 }
 
-// Second-kernel fix-up (see the file comment).  One wavefront per 64 units;
-// the unit where an open row starts owns its sum, computed cooperatively by
+// Second-kernel fix-up (see the file comment).  Work-item t acts for the
+// unit where an open row starts (seg_end[t] >= 0, written by the main
+// kernel): a two-unit row is summed by the work-item itself, longer rows by
 // the whole wavefront with the same fixed order as the in-kernel path.
-__global__ __launch_bounds__(256) void spmv_fixup_kernel(
-    const int* __restrict__ offsets, int rows, long long unit_items,
-    long long num_units, fixup_ws ws, float* __restrict__ y) {
+__global__ __launch_bounds__(256) void spmv_fixup_kernel(long long num_units,
+                                                         fixup_ws ws,
+                                                         float* __restrict__ y) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  long long t0 = -1, t1 = -1;
+  long long t1 = -1;
   int r = -1;
   if (t < num_units) {
-    r = ws.carry_row[t];
-    if (r >= 0 && r < rows) {
-      const long long fr = (long long)offsets[r] + r;
-      if (fr / unit_items == t) {
-        t0 = t;
-        t1 = ((long long)offsets[r + 1] + r + 1 - 1) / unit_items;
-      }
-    }
+    t1 = ws.seg_end[t];
+    if (t1 >= 0) r = ws.carry_row[t];
   }
-  unsigned long long todo = __ballot(t0 >= 0);
+  if (t1 >= 0 && segment_is_short(t, t1))
+    y[r] = short_segment_sum(ws.carry_val, ws.head_val, t, t1);
+  unsigned long long todo = __ballot(t1 >= 0 && !segment_is_short(t, t1));
   while (todo) {
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1;
-    const long long a = __shfl(t0, src, 64);
+    const long long a = __shfl(t, src, 64);
     const long long b = __shfl(t1, src, 64);
     const int rr = __shfl(r, src, 64);
-    const float s = wave_segment_sum(ws.carry_val, ws.head_val, a, b);
-    if (lane == src) y[rr] = s;
+    const float v = wave_segment_sum(ws.carry_val, ws.head_val, a, b);
+    if (lane == src) y[rr] = v;
   }
 }
 
@@ -400,7 +420,8 @@ __global__ __launch_bounds__(TPB) void spmv_work_oriented_kernel(
     const int* __restrict__ offsets, const int* __restrict__ indices,
     const V* __restrict__ values, const float* __restrict__ x,
     float* __restrict__ y, int rows, int nnz, int* __restrict__ carry_row,
-    float* __restrict__ carry_val, float* __restrict__ head_val) {
+    int* __restrict__ seg_end, float* __restrict__ carry_val,
+    float* __restrict__ head_val) {
   using setup_t = loops::schedule::setup<algorithms_t::work_oriented, TPB, 1,
                                          int, int, long long, long long>;
   setup_t config(const_cast<int*>(offsets), rows, nnz);
@@ -424,9 +445,17 @@ __global__ __launch_bounds__(TPB) void spmv_work_oriented_kernel(
       rem += value_traits<V>::load1(values + nz) * x[ld_idx(indices + nz)];
   }
   long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  carry_row[u] = (ex < rows && offsets[ex] < ey) ? ex : -1;
+  const bool open_right = ex < rows && offsets[ex] < ey;
+  carry_row[u] = open_right ? ex : -1;
   carry_val[u] = rem;
   head_val[u] = head;
+  int end_unit = -1;
+  if (open_right) {
+    const long long wpt = config.work_per_thread_count();
+    const long long fr = (long long)offsets[ex] + ex;
+    if (fr / wpt == u) end_unit = (int)(((long long)offsets[ex + 1] + ex) / wpt);
+  }
+  seg_end[u] = end_unit;
 }
 
 // ---------------------------------------------------------------------------
@@ -637,8 +666,10 @@ hipError_t launch_fixup(const int* offsets, int rows, long long unit_items,
   if (num_units <= 0) return hipSuccess;
   const int tpb = 256;
   long long grid = (num_units + tpb - 1) / tpb;
-  spmv_fixup_kernel<<<dim3((unsigned)grid), dim3(tpb), 0, s>>>(
-      offsets, rows, unit_items, num_units, ws, y);
+  (void)offsets;
+  (void)rows;
+  (void)unit_items;
+  spmv_fixup_kernel<<<dim3((unsigned)grid), dim3(tpb), 0, s>>>(num_units, ws, y);
   return hipGetLastError();
 }
 
@@ -671,7 +702,7 @@ hipError_t wo_launch_t(const csr_args& A, const float* x, float* y, int grid,
   if (e != hipSuccess) return e;
   spmv_work_oriented_kernel<TPB, V><<<dim3(grid), dim3(TPB), 0, s>>>(
       A.offsets, A.indices, static_cast<const V*>(A.values), x, y, A.rows,
-      A.nnz, ws.carry_row, ws.carry_val, ws.head_val);
+      A.nnz, ws.carry_row, ws.seg_end, ws.carry_val, ws.head_val);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = record(ev1, s);

@@ -97,9 +97,10 @@ typedef struct loops_coord_t {
  *   group_mapped:    block_mapped<128> (group_mapped.cuh:39,79)
  *   thread_mapped / original: 128 threads, ceil(rows/128) blocks. */
 /* loops_params_t.flags */
-#define LOOPS_FLAG_SEPARATE_FIXUP 1u /* merge_path_flat: finish rows that
-                                        cross tiles in a second kernel
-                                        instead of inside the main kernel */
+#define LOOPS_FLAG_INKERNEL_FIXUP 1u /* merge_path_flat: finish rows that
+                                        cross tiles inside the main kernel
+                                        (agent-scope atomics and fences per
+                                        tile) instead of in a second kernel */
 
 typedef struct loops_params_t {
   int32_t threads_per_block;
@@ -142,9 +143,10 @@ loops_status_t loops_spmv_workspace_size(loops_handle_t handle,
 /* y = A * x (y overwritten).  workspace may be NULL, in which case the
  * handle's own workspace is used (grown with hipMalloc outside any graph
  * capture); a caller workspace needs no initialisation.  merge_path_flat
- * finishes rows that cross tiles inside its kernel with counters owned by
- * the handle (grown outside graph capture; inside a capture that needs more
- * than the handle holds, the separate fix-up kernel is used).  Calls that
+ * finishes rows that cross tiles in a second small kernel; with
+ * LOOPS_FLAG_INKERNEL_FIXUP it does so inside its kernel with counters owned
+ * by the handle (grown outside graph capture; inside a capture that needs
+ * more than the handle holds, the second kernel is used).  Calls that
  * overlap in time must use different handles and workspaces.  Asynchronous
  * on stream. */
 loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,

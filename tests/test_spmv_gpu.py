@@ -121,13 +121,13 @@ def test_merge_path_tunings(tpb, ipt, name):
     y = _run("merge_path_flat", A, x, threads_per_block=tpb, items_per_thread=ipt)
     _check(y, off, idx, val, x, tag=f"mp{tpb}x{ipt}")
     # precomputed tile coordinates give the same bits as the in-kernel search,
-    # and the separate fix-up kernel the same bits as the in-kernel one
+    # and the in-kernel fix-up the same bits as the fix-up kernel
     coords = loops.merge_path_partition(A, tpb * ipt)
     y2 = _run("merge_path_flat", A, x, threads_per_block=tpb, items_per_thread=ipt,
               tile_coordinates=coords)
     assert np.array_equal(y, y2)
     y3 = _run("merge_path_flat", A, x, threads_per_block=tpb, items_per_thread=ipt,
-              separate_fixup=True)
+              inkernel_fixup=True)
     assert np.array_equal(y, y3)
 
 

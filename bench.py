@@ -7,15 +7,18 @@ to 200,000; columns uniform and sorted per row; int32 indices; fp32 values
 and x uniform [-1, 1).  Synthetic data (seeded generator), resident in HBM
 before timing.
 
-A step is one loops_spmv(MERGE_PATH_FLAT) call (main kernel + row fix-up).
-The timed steps rotate over several identical copies of A (working set >
-the 256 MB MALL) so no step is served from the previous step's cache lines;
-the same-copy (warm) figure is reported beside it.
+A step is one execution of the matrix's merge-path SpMV plan
+(loops_spmv_plan_execute: the tiles of 4 column panels of A in one launch,
+the row fix-up, the panel combine), built once per matrix beforehand (its
+build time is reported).  The timed steps rotate over several identical
+copies of A (working set > the 256 MB MALL) so no step is served from the
+previous step's cache lines; the same-copy (warm) figure and the plain
+per-call drop-in (loops_spmv, no plan) are reported beside it.
 
 Prints one JSON line (value = GFLOP/s = 2 nnz / step time) with a roofline
-object for the main merge-path kernel (algorithmic bytes / its average
-duration from HIP events recorded around it on its stream) and the CPU
-baseline (the oracle's sequential validator loop on the host).
+object for the main kernel (algorithmic bytes / its average duration from
+HIP events recorded around it on its stream) and the CPU baseline (the
+oracle's sequential validator loop on the host).
 """
 import argparse
 import json
@@ -43,6 +46,7 @@ def parse():
     ap.add_argument("--copies", type=int, default=4)
     ap.add_argument("--tpb", type=int, default=0)
     ap.add_argument("--ipt", type=int, default=0)
+    ap.add_argument("--panels", type=int, default=0, help="column panels (0: automatic)")
     ap.add_argument("--schedules", default="all",
                     help="comma list of extra schedules to time, 'all' or 'none'")
     ap.add_argument("--cpu-runs", type=int, default=5)
@@ -70,7 +74,7 @@ def copy_calibration(mb, steps):
 def traffic_from_profile():
     """HBM bytes per launch of the merge-path kernel, measured by rocprofv3
     PMC runs and recorded in profiles/ (see DESIGN.md); None if absent."""
-    p = os.path.join(ROOT, "profiles", "merge_path_c2_pmc.md")
+    p = os.path.join(ROOT, "profiles", "merge_path_plan_c2_pmc.md")
     if not os.path.exists(p):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
@@ -112,22 +116,20 @@ def main():
         tuning["threads_per_block"] = args.tpb
     if args.ipt:
         tuning["items_per_thread"] = args.ipt
-    ws = loops.algorithms.workspace("merge_path_flat", copies[0], **tuning)
     torch.cuda.synchronize()
+    t0 = time.time()
+    plans = [loops.spmv_plan(A, panels=args.panels, stream=stream, **tuning) for A in copies]
+    torch.cuda.synchronize()
+    build_ms = (time.time() - t0) * 1e3 / len(plans)
+    log(f"plans: {plans[0].panels} panels, {plans[0].device_bytes/1e6:.0f} MB each, "
+        f"{build_ms:.1f} ms to build")
 
     nbytes = synth.spmv_bytes(R, C, N, 4)
     flops = 2 * N
 
-    def step(i, ev=None):
-        A = copies[i % len(copies)]
-        kw = dict(tuning)
-        if ev is not None:
-            kw["start_event"], kw["stop_event"] = ev
-        loops.run_spmv("merge_path_flat", A, x, y, stream=stream, workspace=ws, **kw)
-
     log(f"warmup {args.warmup}")
     for i in range(args.warmup):
-        step(i)
+        plans[i % len(plans)].execute(x, y, stream)
     torch.cuda.synchronize()
 
     # Timed region: K steps, events around each main kernel.
@@ -136,7 +138,7 @@ def main():
     torch.cuda.synchronize()
     t_start.record(stream)
     for i in range(args.steps):
-        step(i, evs[i])
+        plans[i % len(plans)].execute(x, y, stream, evs[i][0], evs[i][1])
     t_stop.record(stream)
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_ms(t_stop)
@@ -147,39 +149,39 @@ def main():
     log(f"cold: {ms_per_step*1e3:.1f} us/step, kernel {kern_avg*1e3:.1f} us, "
         f"{value:.1f} GFLOP/s, {nbytes/kern_avg/1e6:.0f} GB/s")
 
-    extras = {}
+    extras = {"plan": {"panels": plans[0].panels, "build_ms": round(build_ms, 2),
+                       "device_bytes": plans[0].device_bytes}}
     if not args.quick:
         # warm: the same copy every step
-        for i in range(args.warmup):
-            loops.run_spmv("merge_path_flat", copies[0], x, y, stream=stream, workspace=ws, **tuning)
         w0, w1 = Event(), Event()
         torch.cuda.synchronize()
         w0.record(stream)
         for i in range(args.steps):
-            loops.run_spmv("merge_path_flat", copies[0], x, y, stream=stream, workspace=ws, **tuning)
+            plans[0].execute(x, y, stream)
         w1.record(stream)
         torch.cuda.synchronize()
         warm_ms = w0.elapsed_ms(w1) / args.steps
-        extras["warm_ms_per_step"] = warm_ms
-        extras["warm_gflops"] = flops / (warm_ms * 1e-3) / 1e9
+        extras["warm_ms_per_step"] = round(warm_ms, 5)
         log(f"warm: {warm_ms*1e3:.1f} us/step")
 
-        # with precomputed tile coordinates (partition cached per matrix)
-        ti = (args.tpb or 256) * (args.ipt or 8)
-        coords = [loops.merge_path_partition(A, ti, stream=stream) for A in copies]
-        torch.cuda.synchronize()
+        # the plain per-call drop-in (loops_spmv: in-kernel tile search, no plan)
+        ws = loops.algorithms.workspace("merge_path_flat", copies[0])
+        for i in range(args.warmup):
+            loops.run_spmv("merge_path_flat", copies[i % len(copies)], x, y, stream=stream, workspace=ws)
         p0, p1 = Event(), Event()
+        torch.cuda.synchronize()
         p0.record(stream)
         for i in range(args.steps):
             loops.run_spmv("merge_path_flat", copies[i % len(copies)], x, y, stream=stream,
-                           workspace=ws, tile_coordinates=coords[i % len(copies)], **tuning)
+                           workspace=ws)
         p1.record(stream)
         torch.cuda.synchronize()
-        pre_ms = p0.elapsed_ms(p1) / args.steps
-        extras["precomputed_partition_ms_per_step"] = pre_ms
-        log(f"precomputed partition: {pre_ms*1e3:.1f} us/step")
+        plain_ms = p0.elapsed_ms(p1) / args.steps
+        extras["plain_per_call_ms_per_step"] = round(plain_ms, 5)
+        extras["plain_per_call_gflops"] = round(flops / (plain_ms * 1e-3) / 1e9, 2)
+        log(f"plain per-call loops_spmv: {plain_ms*1e3:.1f} us/step")
 
-        # all five schedules, same matrix (copy 0), a few runs each
+        # all five schedules through loops_spmv, same matrix (copy 0)
         names = ["merge_path_flat", "work_oriented", "group_mapped", "thread_mapped", "original"]
         if args.schedules == "none":
             names = []
@@ -242,7 +244,8 @@ def main():
             "workload": "c2_powerlaw_2m merge_path_flat SpMV",
             "rows": R, "cols": C, "nnz": N, "max_row": A_h.max_row(),
             "zipf": 1.8, "index": "int32", "values": "f32",
-            "threads_per_block": args.tpb or None, "items_per_thread": args.ipt or None,
+            "threads_per_block": args.tpb or 512, "items_per_thread": args.ipt or 4,
+            "column_panels": plans[0].panels,
             "rotated_copies": len(copies), "parallelism": "single-gpu",
         },
         "roofline": {
@@ -252,7 +255,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profile(),
-            "kernel": "spmv_merge_path_kernel",
+            "kernel": "spmv_panels_kernel" if plans[0].panels > 1 else "spmv_merge_path_kernel",
             "kernel_avg_ms": round(kern_avg, 5),
             "algorithmic_bytes": nbytes,
         },

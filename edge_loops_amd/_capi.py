@@ -76,7 +76,15 @@ class Params(ctypes.Structure):
     ]
 
 
-FLAG_INKERNEL_FIXUP = 1
+
+
+class PlanParams(ctypes.Structure):
+    _fields_ = [
+        ("threads_per_block", ctypes.c_int32),
+        ("items_per_thread", ctypes.c_int32),
+        ("panels", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
 
 
 class LoopsError(RuntimeError):
@@ -109,6 +117,11 @@ SIGNATURES = {
     "loops_merge_path_num_tiles": (_st, [_i64, _i64, _i64, ctypes.POINTER(_i64)]),
     "loops_merge_path_partition": (_st, [_vp, _i64, _i64, _i64, _vp, _vp]),
     "loops_work_oriented_partition": (_st, [_vp, _i64, _i64, _i64, _vp, _vp]),
+    "loops_spmv_plan_create": (_st, [_vp, ctypes.POINTER(CsrView), ctypes.POINTER(PlanParams),
+                                     _vp, ctypes.POINTER(ctypes.c_void_p)]),
+    "loops_spmv_plan_execute": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "loops_spmv_plan_info": (_st, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
+    "loops_spmv_plan_destroy": (_st, [_vp]),
     "loops_spmm_workspace_size": (_st, [_vp, _st, ctypes.POINTER(CsrView), _i64,
                                         ctypes.POINTER(Params),
                                         ctypes.POINTER(ctypes.c_size_t)]),

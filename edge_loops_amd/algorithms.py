@@ -34,14 +34,13 @@ def _ptr(t):
 
 def make_params(threads_per_block=0, items_per_thread=0, threads_per_tile=0,
                 grid_size=0, tile_coordinates=None, start_event=None,
-                stop_event=None, inkernel_fixup=False):
+                stop_event=None):
     return _capi.Params(
         int(threads_per_block), int(items_per_thread), int(threads_per_tile),
         int(grid_size),
         _ptr(tile_coordinates) if tile_coordinates is not None else None,
         start_event.ptr if start_event is not None else None,
-        stop_event.ptr if stop_event is not None else None,
-        _capi.FLAG_INKERNEL_FIXUP if inkernel_fixup else 0)
+        stop_event.ptr if stop_event is not None else None, 0)
 
 
 def workspace(schedule, csr, stream=None, **tuning):
@@ -162,6 +161,47 @@ class spmm:
         s = _stream(stream)
         run_spmm("merge_path_flat", csr, B, C, s, **tuning)
         stream_synchronize(s)
+
+
+class spmv_plan:
+    """A merge-path SpMV plan for one matrix (loops_spmv_plan_create): tile
+    coordinates computed once and, with panels > 1, A copied into column
+    panels that run on separate XCDs.  ``execute(x, y)`` is asynchronous and
+    allocates nothing."""
+
+    def __init__(self, csr, panels=0, threads_per_block=0, items_per_thread=0,
+                 stream=None):
+        self.csr = csr  # keeps A alive: with one panel the plan reads it in place
+        self.rows, self.cols = csr.rows, csr.cols
+        v = csr.view()
+        p = _capi.PlanParams(int(threads_per_block), int(items_per_thread), int(panels), 0)
+        h = ctypes.c_void_p()
+        _capi.check(_capi.lib().loops_spmv_plan_create(
+            _capi.handle(), ctypes.byref(v), ctypes.byref(p), _stream(stream),
+            ctypes.byref(h)), "loops_spmv_plan_create")
+        self._h = h
+        n, b = ctypes.c_int32(), ctypes.c_int64()
+        _capi.check(_capi.lib().loops_spmv_plan_info(h, ctypes.byref(n), ctypes.byref(b)),
+                    "loops_spmv_plan_info")
+        self.panels, self.device_bytes = int(n.value), int(b.value)
+
+    def execute(self, x, y, stream=None, start_event=None, stop_event=None):
+        _capi.check(_capi.lib().loops_spmv_plan_execute(
+            self._h, _ptr(x), _ptr(y),
+            start_event.ptr if start_event is not None else None,
+            stop_event.ptr if stop_event is not None else None, _stream(stream)),
+            "loops_spmv_plan_execute")
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            _capi.lib().loops_spmv_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def merge_path_partition(csr, items_per_tile, stream=None):

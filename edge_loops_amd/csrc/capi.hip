@@ -12,13 +12,16 @@
 
 using namespace loops_mi355x;
 
+struct loops_spmv_plan {
+  plan_state st;
+  int64_t rows = 0, cols = 0;
+};
+
 struct loops_handle {
   int device = 0;
   int cu_count = 0;
   void* ws = nullptr;
   size_t ws_bytes = 0;
-  int* counters = nullptr;  // in-kernel fix-up counters, kept all-zero
-  long long counter_cap = 0;
 };
 
 namespace {
@@ -136,36 +139,6 @@ loops_status_t get_workspace(loops_handle_t h, size_t need, void* ws,
   return LOOPS_SUCCESS;
 }
 
-// Counters for the in-kernel fix-up of a call with `units` merge tiles, or
-// nullptr when the handle cannot provide them without allocating inside a
-// graph capture (the caller then uses the fix-up kernel instead).
-int* fixup_counters(loops_handle_t h, long long units, hipStream_t s,
-                    loops_status_t* st) {
-  *st = LOOPS_SUCCESS;
-  if (!h) return nullptr;
-  if (h->counter_cap >= units) return h->counters;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
-    return nullptr;
-  long long cap = units < 65536 ? 65536 : units;
-  int* p = nullptr;
-  if (hipMalloc(&p, (size_t)cap * sizeof(int)) != hipSuccess) {
-    *st = LOOPS_ERROR_ALLOCATION;
-    return nullptr;
-  }
-  hipError_t e = hipMemset(p, 0, (size_t)cap * sizeof(int));
-  if (e == hipSuccess) e = hipDeviceSynchronize();  // old counters idle before free
-  if (e != hipSuccess) {
-    (void)hipFree(p);
-    *st = hip_status(e);
-    return nullptr;
-  }
-  if (h->counters) (void)hipFree(h->counters);
-  h->counters = p;
-  h->counter_cap = cap;
-  return p;
-}
-
 }  // namespace
 
 extern "C" {
@@ -182,12 +155,6 @@ loops_status_t loops_create(loops_handle_t* handle) {
     delete h;
     return hip_status(e);
   }
-  loops_status_t st;
-  fixup_counters(h, 65536, nullptr, &st);
-  if (st != LOOPS_SUCCESS) {
-    delete h;
-    return st;
-  }
   *handle = h;
   return LOOPS_SUCCESS;
 }
@@ -196,10 +163,6 @@ loops_status_t loops_destroy(loops_handle_t handle) {
   if (!handle) return LOOPS_ERROR_INVALID_VALUE;
   hipError_t e = hipSuccess;
   if (handle->ws) e = hipFree(handle->ws);
-  if (handle->counters) {
-    hipError_t e2 = hipFree(handle->counters);
-    if (e == hipSuccess) e = e2;
-  }
   delete handle;
   return hip_status(e);
 }
@@ -271,20 +234,14 @@ loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
   hipEvent_t ev0 = params ? params->start_event : nullptr;
   hipEvent_t ev1 = params ? params->stop_event : nullptr;
   switch (schedule) {
-    case LOOPS_SCHEDULE_MERGE_PATH_FLAT: {
-      int* counters = nullptr;
-      if (params && (params->flags & LOOPS_FLAG_INKERNEL_FIXUP)) {
-        counters = fixup_counters(handle, r.units, stream, &st);
-        if (st != LOOPS_SUCCESS) return st;
-      }
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT:
       return hip_status(launch_mp_spmv(a, x, y, r.tpb, r.ipt,
                                        params ? params->tile_coordinates : nullptr,
-                                       carve_fixup_ws(ws, r.units, counters),
-                                       counters != nullptr, ev0, ev1, stream));
-    }
+                                       carve_fixup_ws(ws, r.units), ev0, ev1,
+                                       stream));
     case LOOPS_SCHEDULE_WORK_ORIENTED:
       return hip_status(launch_wo_spmv(a, x, y, r.tpb, r.grid,
-                                       carve_fixup_ws(ws, r.units, nullptr), ev0,
+                                       carve_fixup_ws(ws, r.units), ev0,
                                        ev1, stream));
     case LOOPS_SCHEDULE_GROUP_MAPPED:
       return hip_status(launch_gm_spmv(a, x, y, r.tpb, r.tpt, ev0, ev1, stream));
@@ -330,6 +287,70 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
   if (rows + nnzs >= (int64_t)INT_MAX) return LOOPS_ERROR_OVERFLOW;
   return hip_status(launch_wo_partition(offsets, (int)rows, (int)nnzs,
                                         num_threads, coords, stream));
+}
+
+loops_status_t loops_spmv_plan_create(loops_handle_t handle,
+                                      const loops_csr_view_t* A,
+                                      const loops_plan_params_t* params,
+                                      hipStream_t stream,
+                                      loops_spmv_plan_t* plan) {
+  (void)handle;
+  if (!plan) return LOOPS_ERROR_INVALID_VALUE;
+  csr_args a;
+  loops_status_t st = check_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  const loops_plan_params_t zero{};
+  const loops_plan_params_t* p = params ? params : &zero;
+  if (p->panels < 0 || p->panels > kMaxPanels || p->reserved != 0 ||
+      p->threads_per_block < 0 || p->items_per_thread < 0)
+    return LOOPS_ERROR_INVALID_VALUE;
+  const int tpb = p->threads_per_block ? p->threads_per_block : 512;
+  const int ipt = p->items_per_thread ? p->items_per_thread : 4;
+  int panels = p->panels;
+  if (panels == 0) {
+    // one panel per 2 MiB of x (half an XCD's 4 MB L2), at most 8
+    long long units = ((long long)a.cols * 4 + (2 << 20) - 1) >> 21;
+    panels = units < 2 ? 1 : (units > kMaxPanels ? kMaxPanels : (int)units);
+  }
+  if (panels > 1 && !panels_supported(tpb, ipt)) return LOOPS_ERROR_NOT_SUPPORTED;
+  if (panels == 1 && !mp_spmv_supported(tpb, ipt)) return LOOPS_ERROR_NOT_SUPPORTED;
+  loops_spmv_plan* pl = new (std::nothrow) loops_spmv_plan();
+  if (!pl) return LOOPS_ERROR_ALLOCATION;
+  pl->st.aligned = a.aligned;
+  pl->rows = A->rows;
+  pl->cols = A->cols;
+  hipError_t e = plan_build(a, panels, tpb, ipt, &pl->st, stream);
+  if (e != hipSuccess) {
+    plan_release(&pl->st);
+    delete pl;
+    return e == hipErrorOutOfMemory ? LOOPS_ERROR_ALLOCATION : hip_status(e);
+  }
+  *plan = pl;
+  return LOOPS_SUCCESS;
+}
+
+loops_status_t loops_spmv_plan_execute(loops_spmv_plan_t plan, const float* x,
+                                       float* y, hipEvent_t start_event,
+                                       hipEvent_t stop_event,
+                                       hipStream_t stream) {
+  if (!plan) return LOOPS_ERROR_INVALID_VALUE;
+  if ((plan->rows > 0 && !y) || (plan->cols > 0 && !x)) return LOOPS_ERROR_INVALID_VALUE;
+  return hip_status(plan_execute(&plan->st, x, y, start_event, stop_event, stream));
+}
+
+loops_status_t loops_spmv_plan_info(loops_spmv_plan_t plan, int32_t* panels,
+                                    int64_t* device_bytes) {
+  if (!plan) return LOOPS_ERROR_INVALID_VALUE;
+  if (panels) *panels = plan->st.P;
+  if (device_bytes) *device_bytes = (int64_t)plan->st.device_bytes;
+  return LOOPS_SUCCESS;
+}
+
+loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan) {
+  if (!plan) return LOOPS_ERROR_INVALID_VALUE;
+  plan_release(&plan->st);
+  delete plan;
+  return LOOPS_SUCCESS;
 }
 
 loops_status_t loops_spmm_workspace_size(loops_handle_t handle,

@@ -24,32 +24,28 @@ struct csr_args {
 
 // Slots of the row fix-up: per work unit (merge tile or work_oriented
 // thread) the row continuing past its end and two partial sums (caller
-// workspace, no initialisation needed).  counter: per unit, used only by the
-// in-kernel fix-up; it lives in memory owned by the handle that holds
-// nothing else, is zero before a call and is left zero by it (the last
-// participant of each row resets its counter).
-// seg_end[u]: for the unit where its open row starts, the unit holding that
-// row's end (so the fix-up needs no offsets lookup); -1 otherwise.
+// workspace, no initialisation needed).  seg_end[u]: for the unit where its
+// open row starts, the unit holding that row's end (so the fix-up needs no
+// offsets lookup); -1 otherwise.
 struct fixup_ws {
   int* carry_row;
   int* seg_end;
   float* carry_val;
   float* head_val;
-  int* counter;
 };
 
 inline size_t fixup_ws_bytes(long long units) {
   size_t b = (size_t)units * (2 * sizeof(int) + 2 * sizeof(float));
   return (b + 255) & ~size_t(255);
 }
-fixup_ws carve_fixup_ws(void* ws, long long units, int* counters);
+fixup_ws carve_fixup_ws(void* ws, long long units);
 
 // ---- SpMV ----------------------------------------------------------------
 bool mp_spmv_supported(int tpb, int ipt);
 hipError_t launch_mp_spmv(const csr_args& A, const float* x, float* y, int tpb,
                           int ipt, const loops_coord_t* coords,
-                          const fixup_ws& ws, bool fused, hipEvent_t ev0,
-                          hipEvent_t ev1, hipStream_t s);
+                          const fixup_ws& ws, hipEvent_t ev0, hipEvent_t ev1,
+                          hipStream_t s);
 hipError_t launch_mp_partition(const int* offsets, int rows, int nnz,
                                long long items_per_tile, long long num_coords,
                                loops_coord_t* coords, hipStream_t s);
@@ -73,6 +69,46 @@ hipError_t launch_gm_spmv(const csr_args& A, const float* x, float* y, int tpb,
 hipError_t launch_tm_spmv(const csr_args& A, const float* x, float* y, int tpb,
                           bool original, hipEvent_t ev0, hipEvent_t ev1,
                           hipStream_t s);
+
+// ---- column-panel plan ---------------------------------------------------
+constexpr int kMaxPanels = 8;
+struct panel_bounds_t {
+  int c[kMaxPanels + 1];  // panel p holds columns [c[p], c[p+1])
+};
+struct panel_ptrs_t {
+  int* p[kMaxPanels];
+};
+struct panel_vptrs_t {
+  void* p[kMaxPanels];
+};
+// Everything one launch needs about every panel (kernel argument).
+struct panel_set_t {
+  const int* offsets[kMaxPanels];
+  const int* indices[kMaxPanels];
+  const void* values[kMaxPanels];
+  const loops_coord_t* coords[kMaxPanels];
+  float* y[kMaxPanels];  // partial outputs (P > 1)
+  fixup_ws ws[kMaxPanels];
+  int nnz[kMaxPanels];
+  int tiles[kMaxPanels];
+};
+struct plan_state {
+  int P = 1, rows = 0, tpb = 256, ipt = 8, tmax = 0;
+  bool aligned = true;
+  loops_value_type_t vtype = LOOPS_VALUE_F32;
+  panel_bounds_t bound{};
+  panel_set_t ps{};
+  void* offsets_mem = nullptr;  // P > 1: panel row offsets
+  void* data_mem = nullptr;     // P > 1: panel column ids and values
+  void* aux_mem = nullptr;      // tile coordinates, fix-up slots, partial y
+  size_t device_bytes = 0;
+};
+bool panels_supported(int tpb, int ipt);
+hipError_t plan_build(const csr_args& A, int P, int tpb, int ipt, plan_state* st,
+                      hipStream_t s);
+hipError_t plan_execute(plan_state* st, const float* x, float* y, hipEvent_t ev0,
+                        hipEvent_t ev1, hipStream_t s);
+void plan_release(plan_state* st);
 
 // ---- SpMM ----------------------------------------------------------------
 hipError_t launch_tm_spmm(const csr_args& A, int k, const float* B, long long ldb,

@@ -96,12 +96,6 @@ typedef struct loops_coord_t {
  *   work_oriented:   128 threads, grid 2 x CU count (work_oriented.cuh:97-104)
  *   group_mapped:    block_mapped<128> (group_mapped.cuh:39,79)
  *   thread_mapped / original: 128 threads, ceil(rows/128) blocks. */
-/* loops_params_t.flags */
-#define LOOPS_FLAG_INKERNEL_FIXUP 1u /* merge_path_flat: finish rows that
-                                        cross tiles inside the main kernel
-                                        (agent-scope atomics and fences per
-                                        tile) instead of in a second kernel */
-
 typedef struct loops_params_t {
   int32_t threads_per_block;
   int32_t items_per_thread;  /* merge_path_flat only                       */
@@ -115,7 +109,7 @@ typedef struct loops_params_t {
    * util::timer_t returned by merge_path_flat (merge_path_flat.cuh:121-138). */
   hipEvent_t start_event;
   hipEvent_t stop_event;
-  uint32_t flags;
+  uint32_t flags; /* reserved, must be 0 */
 } loops_params_t;
 
 typedef struct loops_handle* loops_handle_t;
@@ -143,12 +137,9 @@ loops_status_t loops_spmv_workspace_size(loops_handle_t handle,
 /* y = A * x (y overwritten).  workspace may be NULL, in which case the
  * handle's own workspace is used (grown with hipMalloc outside any graph
  * capture); a caller workspace needs no initialisation.  merge_path_flat
- * finishes rows that cross tiles in a second small kernel; with
- * LOOPS_FLAG_INKERNEL_FIXUP it does so inside its kernel with counters owned
- * by the handle (grown outside graph capture; inside a capture that needs
- * more than the handle holds, the second kernel is used).  Calls that
- * overlap in time must use different handles and workspaces.  Asynchronous
- * on stream. */
+ * and work_oriented finish rows that cross work units in a second small
+ * kernel.  Calls that overlap in time must use different workspaces.
+ * Asynchronous on stream. */
 loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
                           const loops_csr_view_t* A, const float* x, float* y,
                           const loops_params_t* params, void* workspace,
@@ -176,6 +167,38 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
                                              int64_t num_threads,
                                              loops_coord_t* coords,
                                              hipStream_t stream);
+
+/* ---- SpMV plan: preprocessed once per matrix, executed many times ------
+ * merge_path_flat SpMV with the tile coordinates computed once (the
+ * reference's preprocess_t, merge_path_flat.hxx:150-230) and, with panels >
+ * 1, A copied into `panels` column panels whose tiles run on separate XCDs
+ * so each XCD's L2 holds only its slice of x (no counterpart in the
+ * reference).  Creation allocates and synchronises `stream`; execution
+ * allocates nothing and can be captured into a hipGraph.  With panels == 1
+ * the plan reads A in place, so A must outlive it unchanged; with panels > 1
+ * A is no longer needed after creation. */
+typedef struct loops_spmv_plan* loops_spmv_plan_t;
+
+typedef struct loops_plan_params_t {
+  int32_t threads_per_block; /* 0: 512                                      */
+  int32_t items_per_thread;  /* 0: 4                                        */
+  int32_t panels;            /* 1..8; 0: one per 2 MiB of x, at most 8      */
+  int32_t reserved;          /* must be 0                                   */
+} loops_plan_params_t;
+
+loops_status_t loops_spmv_plan_create(loops_handle_t handle,
+                                      const loops_csr_view_t* A,
+                                      const loops_plan_params_t* params,
+                                      hipStream_t stream,
+                                      loops_spmv_plan_t* plan);
+/* y = A * x (y overwritten); events (optional) around the main kernel. */
+loops_status_t loops_spmv_plan_execute(loops_spmv_plan_t plan, const float* x,
+                                       float* y, hipEvent_t start_event,
+                                       hipEvent_t stop_event,
+                                       hipStream_t stream);
+loops_status_t loops_spmv_plan_info(loops_spmv_plan_t plan, int32_t* panels,
+                                    int64_t* device_bytes);
+loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
 
 /* SpMM, C = A * B with B (cols x k) and C (rows x k) dense row-major fp32
  * (container/matrix.cuh:27-35) with leading dimensions ldb, ldc >= k.

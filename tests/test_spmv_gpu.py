@@ -120,15 +120,11 @@ def test_merge_path_tunings(tpb, ipt, name):
     A = _dev(off, idx, val, cols)
     y = _run("merge_path_flat", A, x, threads_per_block=tpb, items_per_thread=ipt)
     _check(y, off, idx, val, x, tag=f"mp{tpb}x{ipt}")
-    # precomputed tile coordinates give the same bits as the in-kernel search,
-    # and the in-kernel fix-up the same bits as the fix-up kernel
+    # precomputed tile coordinates give the same bits as the in-kernel search
     coords = loops.merge_path_partition(A, tpb * ipt)
     y2 = _run("merge_path_flat", A, x, threads_per_block=tpb, items_per_thread=ipt,
               tile_coordinates=coords)
     assert np.array_equal(y, y2)
-    y3 = _run("merge_path_flat", A, x, threads_per_block=tpb, items_per_thread=ipt,
-              inkernel_fixup=True)
-    assert np.array_equal(y, y3)
 
 
 @pytest.mark.parametrize("tpb,tpt", [(64, 64), (128, 128), (256, 256), (128, 64), (256, 64),

@@ -87,6 +87,29 @@ class PlanParams(ctypes.Structure):
     ]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("A", CsrView),
+        ("k", ctypes.c_int64),
+        ("x", ctypes.c_void_p),
+        ("ldx", ctypes.c_int64),
+        ("y", ctypes.c_void_p),
+        ("ldy", ctypes.c_int64),
+        ("schedule", ctypes.c_int),
+        ("items_per_thread", ctypes.c_int32),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("table_bytes", ctypes.c_int64),
+        ("workspace_bytes", ctypes.c_int64),
+        ("work_items", ctypes.c_int64),
+        ("spmv_tiles", ctypes.c_int64),
+        ("spmm_units", ctypes.c_int64),
+    ]
+
+
 class LoopsError(RuntimeError):
     """Raised for a non-success loops_status_t (the C++ layer's
     error::exception_t, reference include/loops/error.hxx:21-45)."""
@@ -122,6 +145,10 @@ SIGNATURES = {
     "loops_spmv_plan_execute": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "loops_spmv_plan_info": (_st, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
     "loops_spmv_plan_destroy": (_st, [_vp]),
+    "loops_batch_query": (_st, [ctypes.POINTER(BatchItem), _i64, ctypes.POINTER(BatchInfo)]),
+    "loops_batch_prepare": (_st, [ctypes.POINTER(BatchItem), _i64, _vp, ctypes.c_size_t, _vp,
+                                  ctypes.c_size_t, ctypes.POINTER(BatchInfo)]),
+    "loops_batch_launch": (_st, [_vp, ctypes.POINTER(BatchInfo), _i32, _vp]),
     "loops_spmm_workspace_size": (_st, [_vp, _st, ctypes.POINTER(CsrView), _i64,
                                         ctypes.POINTER(Params),
                                         ctypes.POINTER(ctypes.c_size_t)]),

@@ -353,6 +353,44 @@ loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan) {
   return LOOPS_SUCCESS;
 }
 
+loops_status_t loops_batch_query(const loops_batch_item_t* items, int64_t n,
+                                 loops_batch_info_t* info) {
+  if (!info || n < 0 || (n > 0 && !items) || n >= INT_MAX) return LOOPS_ERROR_INVALID_VALUE;
+  return batch_layout(items, n, info, nullptr, nullptr);
+}
+
+loops_status_t loops_batch_prepare(const loops_batch_item_t* items, int64_t n,
+                                   void* device_workspace,
+                                   size_t workspace_bytes, void* host_table,
+                                   size_t table_bytes,
+                                   loops_batch_info_t* info) {
+  if (!info || !host_table || n < 0 || (n > 0 && !items) || n >= INT_MAX)
+    return LOOPS_ERROR_INVALID_VALUE;
+  loops_status_t st = batch_layout(items, n, info, nullptr, nullptr);
+  if (st != LOOPS_SUCCESS) return st;
+  if ((int64_t)table_bytes < info->table_bytes) return LOOPS_ERROR_INVALID_VALUE;
+  if (!device_workspace || (int64_t)workspace_bytes < info->workspace_bytes)
+    return LOOPS_ERROR_WORKSPACE;
+  return batch_layout(items, n, info, static_cast<batch_item_dev*>(host_table),
+                      static_cast<char*>(device_workspace));
+}
+
+loops_status_t loops_batch_launch(const void* device_table,
+                                  const loops_batch_info_t* info, int32_t grid,
+                                  hipStream_t stream) {
+  if (!info || (!device_table && info->work_items > 0) || grid < 0)
+    return LOOPS_ERROR_INVALID_VALUE;
+  if (grid == 0) {
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess)
+      e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return hip_status(e);
+    grid = 4 * cus;
+  }
+  return hip_status(batch_launch(device_table, info, grid, stream));
+}
+
 loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
                                          loops_schedule_t schedule,
                                          const loops_csr_view_t* A, int64_t k,

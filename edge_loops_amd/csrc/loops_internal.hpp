@@ -110,6 +110,14 @@ hipError_t plan_execute(plan_state* st, const float* x, float* y, hipEvent_t ev0
                         hipEvent_t ev1, hipStream_t s);
 void plan_release(plan_state* st);
 
+// ---- batches -------------------------------------------------------------
+struct batch_item_dev;
+loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
+                            loops_batch_info_t* info, batch_item_dev* out_table,
+                            char* dev_ws);
+hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
+                        int grid, hipStream_t s);
+
 // ---- SpMM ----------------------------------------------------------------
 hipError_t launch_tm_spmm(const csr_args& A, int k, const float* B, long long ldb,
                           float* C, long long ldc, int tpb, hipEvent_t ev0,

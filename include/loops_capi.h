@@ -200,6 +200,54 @@ loops_status_t loops_spmv_plan_info(loops_spmv_plan_t plan, int32_t* panels,
                                     int64_t* device_bytes);
 loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
 
+/* ---- Batches: many independent products in one persistent launch -------
+ * No counterpart in the reference.  Each item is y = A x (k == 1) or
+ * Y = A X (k > 1, X cols x k and Y rows x k row-major), computed exactly as
+ * it would be alone with the same schedule and tile size.  Items may use:
+ *   k == 1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge tiles of 256 x
+ *           items_per_thread steps (4, 8 or 16; 0 = 8);
+ *   k == 1, LOOPS_SCHEDULE_THREAD_MAPPED: one row per lane;
+ *   k >  1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge-path units of
+ *           items_per_thread steps (0 = 256) walked by lane groups spanning
+ *           the k columns.
+ * Values fp32 or bf16; indices / values 16-byte aligned (8 for bf16).
+ * Steps: loops_batch_query (sizes) -> loops_batch_prepare (host only: fills
+ * a table in host memory that points into the device workspace) -> copy
+ * the table to device memory -> loops_batch_launch (kernels only; no
+ * allocation; capturable in a hipGraph; a launch may be replayed as long
+ * as the items' pointers stay valid).  Overlapping launches need their own
+ * table and workspace. */
+typedef struct loops_batch_item_t {
+  loops_csr_view_t A;
+  int64_t k;
+  const float* x;
+  int64_t ldx;
+  float* y;
+  int64_t ldy;
+  loops_schedule_t schedule;
+  int32_t items_per_thread;
+} loops_batch_item_t;
+
+typedef struct loops_batch_info_t {
+  int64_t table_bytes;     /* host table = device table size            */
+  int64_t workspace_bytes; /* device workspace (no initialisation)      */
+  int64_t work_items;      /* workgroup-sized pieces                    */
+  int64_t spmv_tiles;      /* merge tiles of the SpMV items             */
+  int64_t spmm_units;      /* merge-path units of the SpMM items        */
+} loops_batch_info_t;
+
+loops_status_t loops_batch_query(const loops_batch_item_t* items, int64_t n,
+                                 loops_batch_info_t* info);
+loops_status_t loops_batch_prepare(const loops_batch_item_t* items, int64_t n,
+                                   void* device_workspace,
+                                   size_t workspace_bytes, void* host_table,
+                                   size_t table_bytes,
+                                   loops_batch_info_t* info);
+/* grid: persistent workgroups (0: 4 per CU of the current device). */
+loops_status_t loops_batch_launch(const void* device_table,
+                                  const loops_batch_info_t* info, int32_t grid,
+                                  hipStream_t stream);
+
 /* SpMM, C = A * B with B (cols x k) and C (rows x k) dense row-major fp32
  * (container/matrix.cuh:27-35) with leading dimensions ldb, ldc >= k.
  * Schedules: THREAD_MAPPED (algorithms/spmm/thread_mapped.cuh:28-53),

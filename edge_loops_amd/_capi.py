@@ -107,6 +107,9 @@ class BatchInfo(ctypes.Structure):
         ("work_items", ctypes.c_int64),
         ("spmv_tiles", ctypes.c_int64),
         ("spmm_units", ctypes.c_int64),
+        ("spmm_work_items", ctypes.c_int64),
+        ("coordinates", ctypes.c_int64),
+        ("max_items_per_thread", ctypes.c_int64),
     ]
 
 

@@ -371,6 +371,10 @@ loops_status_t loops_batch_prepare(const loops_batch_item_t* items, int64_t n,
   if ((int64_t)table_bytes < info->table_bytes) return LOOPS_ERROR_INVALID_VALUE;
   if (!device_workspace || (int64_t)workspace_bytes < info->workspace_bytes)
     return LOOPS_ERROR_WORKSPACE;
+  // Resident grid sizes of the current device, queried here (outside any
+  // graph capture) and kept for the launches.
+  hipError_t e = batch_warm_up();
+  if (e != hipSuccess) return hip_status(e);
   return batch_layout(items, n, info, static_cast<batch_item_dev*>(host_table),
                       static_cast<char*>(device_workspace));
 }
@@ -380,14 +384,6 @@ loops_status_t loops_batch_launch(const void* device_table,
                                   hipStream_t stream) {
   if (!info || (!device_table && info->work_items > 0) || grid < 0)
     return LOOPS_ERROR_INVALID_VALUE;
-  if (grid == 0) {
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess)
-      e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return hip_status(e);
-    grid = 4 * cus;
-  }
   return hip_status(batch_launch(device_table, info, grid, stream));
 }
 

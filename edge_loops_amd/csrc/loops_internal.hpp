@@ -117,6 +117,7 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
                             char* dev_ws);
 hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
                         int grid, hipStream_t s);
+hipError_t batch_warm_up();
 
 // ---- SpMM ----------------------------------------------------------------
 hipError_t launch_tm_spmm(const csr_args& A, int k, const float* B, long long ldb,

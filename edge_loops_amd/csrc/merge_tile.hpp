@@ -125,16 +125,9 @@ __device__ __forceinline__ void mp_tile(
   const int tile_nnz = ye - ys;
   const int tile_items = (int)(d1 - d0);
 
-  // 2a. row ends of rows xs .. xe, relative to ys; plus the start of row xs
-  for (int i = tid; i <= tile_rows; i += TPB) {
-    int r = xs + 1 + i;
-    if (r > rows) r = rows;
-    s_rowend[i] = offsets[r] - ys;
-  }
-  if (tid == 0) s_coord[4] = offsets[xs] - ys;
-
-  // 2b. products of the tile's nonzeros [ys, ye): every column id and value
-  // load first, then the x gathers, then the LDS stores.
+  // 2. Column ids and values of the tile's nonzeros [ys, ye) are loaded
+  // first, the row ends next (both in flight together), then the x gathers
+  // and the LDS stores of the products.
   const int ybase = ys & ~3;
   const int pofs = ys - ybase;
   {
@@ -156,6 +149,14 @@ __device__ __forceinline__ void mp_tile(
         }
       }
     }
+    // 2a. row ends of rows xs .. xe, relative to ys; plus the start of row xs
+    for (int i = tid; i <= tile_rows; i += TPB) {
+      int r = xs + 1 + i;
+      if (r > rows) r = rows;
+      s_rowend[i] = offsets[r] - ys;
+    }
+    if (tid == 0) s_coord[4] = offsets[xs] - ys;
+
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
       const int j = ybase + 4 * (tid + q * TPB);

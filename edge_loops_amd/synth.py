@@ -111,3 +111,42 @@ def spmv_bytes(rows, cols, nnz, value_bytes=4):
 
 def spmm_bytes(rows, cols, nnz, k, value_bytes=4):
     return nnz * (4 + value_bytes) + 4 * (rows + 1) + 4 * k * cols + 4 * k * rows
+
+
+def c4_batch_shapes(count=512, seed=4):
+    """Row counts of the batched workload: square matrices with rows
+    log-uniform in [1e3, 2e5]."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(1e3), np.log(2e5), count)).astype(np.int64)
+
+
+def c4_batch_matrix(i, rows, seed=4):
+    """Matrix i of the batched workload: nnz/row Zipf(1.8) on [1, 4096]
+    (unscaled), uniform sorted distinct columns, fp32 uniform[-1, 1]."""
+    rows = int(rows)
+    return powerlaw_csr(rows, rows, 1.8, 1, min(4096, rows), 0, seed=seed * 100_000 + i)
+
+
+def c4_batch_512(count=512, seed=4):
+    return [c4_batch_matrix(i, r, seed) for i, r in enumerate(c4_batch_shapes(count, seed))]
+
+
+def c5_pool_shapes(count=64, seed=5):
+    """(rows, target nnz, Zipf exponent) of the served pool: rows log-uniform
+    in [1e4, 4e6], nnz log-uniform in [1e5, 8e6] (at least one per row),
+    exponent uniform in [1.8, 2.2]."""
+    rng = np.random.default_rng(seed)
+    rows = np.exp(rng.uniform(np.log(1e4), np.log(4e6), count)).astype(np.int64)
+    nnz = np.exp(rng.uniform(np.log(1e5), np.log(8e6), count)).astype(np.int64)
+    nnz = np.maximum(nnz, rows)
+    s = rng.uniform(1.8, 2.2, count)
+    return [(int(r), int(n), float(z)) for r, n, z in zip(rows, nnz, s)]
+
+
+def c5_pool_matrix(i, shape, seed=5):
+    rows, nnz, s = shape
+    return powerlaw_csr(rows, rows, s, 1, min(100_000, rows), nnz, seed=seed * 100_000 + i)
+
+
+def c5_pool_64(count=64, seed=5):
+    return [c5_pool_matrix(i, sh, seed) for i, sh in enumerate(c5_pool_shapes(count, seed))]

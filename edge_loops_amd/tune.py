@@ -1,0 +1,253 @@
+"""Autotuner for batch items: picks the schedule and tile size per
+(size, mean row length, row skew, k) bucket from measurements on the GPU.
+
+No counterpart in the reference, whose schedule picks exist only as data
+(plots/data/heuristics.csv: per-matrix best of merge-path / thread-mapped /
+group-mapped).  Here the candidates are the batch kernel's piece kinds:
+
+  k == 1: merge_path_flat with 4, 8 or 16 steps per lane; thread_mapped;
+          group_mapped with G lanes per row (G from the mean row length, and
+          4x that);
+  k >  1: merge_path_flat SpMM units of 64 .. 1024 merge steps.
+
+For each bucket the tuning matrices that fall in it are run as one batch per
+candidate (items repeated until the batch holds enough work to fill the
+GPU), captured in a graph and replayed; the fastest candidate wins.  The
+table is plain JSON: bucket key -> {"schedule", "tile", "us": {candidate:
+microseconds}}.  Lookup falls back to the nearest tuned bucket, then to
+merge_path_flat with 8 steps per lane (SpMM: 256 steps per unit).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+SPMV_MERGE = [4, 8, 16]
+SPMM_IPU = [64, 128, 256, 512, 1024]
+DEFAULT_SPMV = ("merge_path_flat", 8)
+DEFAULT_SPMM = ("merge_path_flat", 256)
+TABLE_PATH = os.path.join(os.path.dirname(__file__), "tuning", "mi355x_batch.json")
+
+
+def features(rows, nnz, max_row, k=1):
+    """Bucket key of a product: 'k{kb}/s{size}/d{density}/w{skew}'.
+    size = floor(log2(rows + nnz)) // 2 (two-octave bins), density =
+    floor(log2(mean row length)) clamped to 0..8, skew = floor(log2(max row /
+    mean row)) // 2 clamped to 0..6, kb = 0 for k == 1 else ceil(log2 k)."""
+    rows, nnz, max_row, k = int(rows), int(nnz), int(max_row), int(k)
+    size = int(math.log2(max(rows + nnz, 1))) // 2
+    mean = nnz / rows if rows else 0.0
+    dens = min(max(int(math.floor(math.log2(mean))) if mean >= 1 else 0, 0), 8)
+    skew_r = max_row / mean if mean > 0 else 1.0
+    skew = min(max(int(math.floor(math.log2(max(skew_r, 1.0)))) // 2, 0), 6)
+    kb = 0 if k <= 1 else min(int(math.ceil(math.log2(k))), 7)
+    return f"k{kb}/s{size}/d{dens}/w{skew}"
+
+
+def _key_vec(key):
+    return np.array([int(p[1:]) for p in key.split("/")])
+
+
+def group_lanes(rows, nnz):
+    mean = (nnz + rows - 1) // rows if rows else 1
+    g = 2
+    while g < 64 and g < mean:
+        g *= 2
+    return g
+
+
+def spmv_candidates():
+    """(schedule, tile) names; group_mapped tiles are lane-count factors
+    over group_lanes() of the matrix ("x1", "x4")."""
+    return ([("merge_path_flat", i) for i in SPMV_MERGE]
+            + [("thread_mapped", 0), ("group_mapped", "x1"), ("group_mapped", "x4")])
+
+
+def resolve(sched, tile, rows, nnz):
+    """items_per_thread for the batch C ABI from a (schedule, tile) name."""
+    if sched == "group_mapped":
+        f = int(str(tile).lstrip("x"))
+        return min(64, group_lanes(rows, nnz) * f)
+    return int(tile)
+
+
+def spmm_candidates():
+    return [("merge_path_flat", u) for u in SPMM_IPU]
+
+
+class Table:
+    """Tuning table with nearest-bucket lookup."""
+
+    def __init__(self, data=None):
+        self.data = data or {"spmv": {}, "spmm": {}}
+
+    @classmethod
+    def load(cls, path=TABLE_PATH):
+        if not os.path.exists(path):
+            return cls()
+        with open(path) as f:
+            return cls(json.load(f))
+
+    def save(self, path=TABLE_PATH):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(self.data, f, indent=1, sort_keys=True)
+
+    def pick(self, rows, nnz, max_row, k=1):
+        """(schedule, items_per_thread) for one product."""
+        sched, tile = self.pick_name(rows, nnz, max_row, k)
+        return sched, resolve(sched, tile, rows, nnz)
+
+    def pick_name(self, rows, nnz, max_row, k=1):
+        key = features(rows, nnz, max_row, k)
+        tab = self.data["spmv" if k == 1 else "spmm"]
+        if key in tab:
+            e = tab[key]
+        elif tab:
+            kv = _key_vec(key)
+            near = min(tab, key=lambda q: (int(np.abs(_key_vec(q) - kv).sum()), q))
+            e = tab[near]
+        else:
+            return DEFAULT_SPMV if k == 1 else DEFAULT_SPMM
+        # JSON keeps tiles as strings ("8", "x4"); numeric ones become int.
+        t = e["tile"]
+        return e["schedule"], int(t) if str(t).isdigit() else t
+
+
+def tuning_set(count=192, seed=77):
+    """Host matrices covering the buckets the batched and served workloads
+    hit: rows log-uniform in [1e3, 4e6], mean row length log-uniform in
+    [1, 48] (half unscaled Zipf), Zipf exponent uniform in [1.5, 2.5],
+    at most 8M nonzeros each."""
+    from . import synth
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        rows = int(math.exp(rng.uniform(math.log(1e3), math.log(4e6))))
+        s = float(rng.uniform(1.5, 2.5))
+        cap = min(4096, rows)
+        if i % 2 == 0:
+            target = 0
+        else:
+            mean = math.exp(rng.uniform(0, math.log(48)))
+            target = int(min(max(rows * mean, rows), 8e6))
+        m = synth.powerlaw_csr(rows, rows, s, 1, cap, target, seed=seed * 1000 + i)
+        if m.nnz > 8e6:
+            continue
+        out.append(m)
+    return out
+
+
+def _time_batch(items, reps=5):
+    """Microseconds per replay of a graph holding one batch launch."""
+    import torch
+    from .batch import Batch
+    b = Batch(items)
+    g = b.capture()
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1000.0 / reps
+
+
+def tune(mats, ks=(2, 4, 8, 16, 32), min_work=30_000_000, log=print):
+    """Measure every candidate per bucket on the GPU; returns a Table."""
+    import torch
+    from . import containers
+    from .batch import BatchItem
+    dev = []
+    for m in mats:
+        A = containers.csr_t.from_host(m.rows, m.cols, m.offsets, m.indices, m.values)
+        dev.append((m, A))
+    table = Table()
+    # SpMV buckets
+    buckets = {}
+    for m, A in dev:
+        buckets.setdefault(features(m.rows, m.nnz, m.max_row(), 1), []).append((m, A))
+    xs = {}
+    for key in sorted(buckets):
+        group = buckets[key]
+        work = sum(m.rows + m.nnz for m, _ in group)
+        rep = max(1, math.ceil(min_work / max(work, 1)))
+        rep = min(rep, 64)
+        times = {}
+        for sched, tile in spmv_candidates():
+            items = []
+            for r in range(rep):
+                for m, A in group:
+                    if id(A) not in xs:
+                        xs[id(A)] = torch.rand(A.cols, device="cuda") * 2 - 1
+                    y = torch.empty(A.rows, device="cuda")
+                    items.append(BatchItem(A, xs[id(A)], y, sched,
+                                           resolve(sched, tile, m.rows, m.nnz)))
+            times[f"{sched}:{tile}"] = _time_batch(items)
+        best = min(times, key=times.get)
+        sched, tile = best.split(":")
+        table.data["spmv"][key] = {"schedule": sched, "tile": tile, "us": times,
+                                   "matrices": len(group), "repeat": rep}
+        log(f"spmv {key}: n={len(group)} rep={rep} best={best} "
+            + " ".join(f"{c}={t:.1f}" for c, t in sorted(times.items())))
+    # SpMM buckets (a subset of matrices per bucket keeps the run short)
+    for k in ks:
+        buckets = {}
+        for m, A in dev:
+            buckets.setdefault(features(m.rows, m.nnz, m.max_row(), k), []).append((m, A))
+        for key in sorted(buckets):
+            group = buckets[key][:6]
+            work = sum((m.rows + m.nnz) * k for m, _ in group)
+            rep = min(max(1, math.ceil(min_work * 4 / max(work, 1))), 32)
+            times = {}
+            X = {}
+            for sched, ipu in spmm_candidates():
+                items = []
+                for r in range(rep):
+                    for m, A in group:
+                        if id(A) not in X:
+                            X[id(A)] = torch.rand(A.cols, k, device="cuda") * 2 - 1
+                        Y = torch.empty(A.rows, k, device="cuda")
+                        items.append(BatchItem(A, X[id(A)], Y, sched, ipu))
+                times[f"{sched}:{ipu}"] = _time_batch(items)
+            best = min(times, key=times.get)
+            sched, tile = best.split(":")
+            table.data["spmm"][key] = {"schedule": sched, "tile": tile, "us": times,
+                                       "matrices": len(group), "repeat": rep}
+            log(f"spmm {key}: n={len(group)} rep={rep} best={best} "
+                + " ".join(f"{c}={t:.1f}" for c, t in sorted(times.items())))
+            del X
+            torch.cuda.empty_cache()
+    return table
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="Tune batch schedules on this GPU")
+    ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--count", type=int, default=192)
+    ap.add_argument("--quick", action="store_true")
+    a = ap.parse_args(argv)
+    mats = tuning_set(24 if a.quick else a.count)
+    print(f"tuning set: {len(mats)} matrices, {sum(m.nnz for m in mats)} nnz", flush=True)
+    t = tune(mats, ks=(4, 32) if a.quick else (2, 4, 8, 16, 32),
+             log=lambda s: print(s, flush=True))
+    t.data["about"] = {
+        "device": "MI355X (gfx950)",
+        "features": "k{ceil log2 k}/s{floor log2(rows+nnz) // 2}/d{floor log2 mean row}"
+                    "/w{floor log2(max row / mean row) // 2}",
+        "tuning_set": f"{len(mats)} matrices from edge_loops_amd.tune.tuning_set",
+        "command": "python -m edge_loops_amd.tune --out <path>",
+    }
+    t.save(a.out)
+    print("wrote", a.out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

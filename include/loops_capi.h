@@ -207,10 +207,14 @@ loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
  *   k == 1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge tiles of 256 x
  *           items_per_thread steps (4, 8 or 16; 0 = 8);
  *   k == 1, LOOPS_SCHEDULE_THREAD_MAPPED: one row per lane;
+ *   k == 1, LOOPS_SCHEDULE_GROUP_MAPPED: items_per_thread lanes per row
+ *           (2, 4, ..., 64; 0 = the power of two at or above the mean row
+ *           length), reduced in a fixed order;
  *   k >  1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge-path units of
- *           items_per_thread steps (0 = 256) walked by lane groups spanning
- *           the k columns.
- * Values fp32 or bf16; indices / values 16-byte aligned (8 for bf16).
+ *           items_per_thread steps (16..65536; 0 = 256) walked by lane
+ *           groups spanning the k columns.
+ * Values fp32, bf16 or fp16; indices / values 16-byte aligned (8 for 16-bit
+ * values).
  * Steps: loops_batch_query (sizes) -> loops_batch_prepare (host only: fills
  * a table in host memory that points into the device workspace) -> copy
  * the table to device memory -> loops_batch_launch (kernels only; no
@@ -234,6 +238,9 @@ typedef struct loops_batch_info_t {
   int64_t work_items;      /* workgroup-sized pieces                    */
   int64_t spmv_tiles;      /* merge tiles of the SpMV items             */
   int64_t spmm_units;      /* merge-path units of the SpMM items        */
+  int64_t spmm_work_items; /* pieces of the SpMM items (of work_items)  */
+  int64_t coordinates;     /* merge-tile coordinates written per launch */
+  int64_t max_items_per_thread; /* largest merge tile of the SpMV items   */
 } loops_batch_info_t;
 
 loops_status_t loops_batch_query(const loops_batch_item_t* items, int64_t n,
@@ -243,7 +250,9 @@ loops_status_t loops_batch_prepare(const loops_batch_item_t* items, int64_t n,
                                    size_t workspace_bytes, void* host_table,
                                    size_t table_bytes,
                                    loops_batch_info_t* info);
-/* grid: persistent workgroups (0: 4 per CU of the current device). */
+/* grid: persistent workgroups per kernel (0: as many as are resident at
+ * once on the current device).  loops_batch_prepare also queries the
+ * current device's occupancy (once per device). */
 loops_status_t loops_batch_launch(const void* device_table,
                                   const loops_batch_info_t* info, int32_t grid,
                                   hipStream_t stream);

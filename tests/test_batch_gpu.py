@@ -36,7 +36,9 @@ SPEC = [(0, 1, "merge_path_flat", 4), (1, 1, "merge_path_flat", 8), (2, 1, "merg
         (3, 1, "thread_mapped", 0), (4, 2, "merge_path_flat", 0), (5, 5, "merge_path_flat", 64),
         (6, 1, "merge_path_flat", 8), (6, 16, "merge_path_flat", 256), (1, 33, "merge_path_flat", 128),
         (2, 64, "merge_path_flat", 0), (0, 100, "merge_path_flat", 512), (7, 1, "merge_path_flat", 8),
-        (7, 3, "merge_path_flat", 0), (6, 1, "thread_mapped", 0)]
+        (7, 3, "merge_path_flat", 0), (6, 1, "thread_mapped", 0), (3, 1, "group_mapped", 0),
+        (6, 1, "group_mapped", 64), (1, 1, "group_mapped", 2), (7, 1, "group_mapped", 0),
+        (4, 1, "group_mapped", 16), (6, 32, "merge_path_flat", 64)]
 
 
 def _items(vt="f32", seed=0):
@@ -63,7 +65,7 @@ def _ref(m, xh, stored=None):
     return oracle.spmm_f64(off, idx, v, xh)
 
 
-@pytest.mark.parametrize("vt", ["f32", "bf16"])
+@pytest.mark.parametrize("vt", ["f32", "bf16", "f16"])
 def test_batch_parity_alone_and_in_mix(vt):
     its = _items(vt)
     b = Batch([i for i, _, _ in its])
@@ -123,7 +125,7 @@ def test_batch_graph_replay():
             assert np.all(np.abs(yr - r64) <= 1e-5 * a64 + 1e-30)
 
 
-def test_batch_rejects_unaligned_and_f16():
+def test_batch_rejects_unaligned_and_bad_groups():
     off, idx, val, cols = MATS[0]
     idx_d = torch.from_numpy(np.concatenate([[0], idx]).astype(np.int32)).cuda()[1:]
     val_d = torch.from_numpy(np.concatenate([[0], val]).astype(np.float32)).cuda()[1:]
@@ -132,6 +134,9 @@ def test_batch_rejects_unaligned_and_f16():
     y = torch.zeros(A.rows, device="cuda")
     with pytest.raises(loops.LoopsError):
         Batch([BatchItem(A, x, y)])
-    A16 = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val, value_type="f16")
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+    for bad in (3, 128):
+        with pytest.raises(loops.LoopsError):
+            Batch([BatchItem(A, x, y, "group_mapped", bad)])
     with pytest.raises(loops.LoopsError):
-        Batch([BatchItem(A16, x, y)])
+        Batch([BatchItem(A, x, y, "work_oriented")])

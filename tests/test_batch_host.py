@@ -1,0 +1,84 @@
+"""Host-side checks of the batch layout (loops_batch_query needs no GPU) and
+of the autotuner's bucketing and table lookup."""
+import ctypes
+
+import pytest
+
+from edge_loops_amd import _capi, tune
+
+BASE = 1 << 20  # fake, 256-byte aligned device addresses (never dereferenced)
+
+
+def item(rows, nnz, k=1, schedule="merge_path_flat", ipt=0, vt=_capi.VALUE_F32,
+         idx_off=0, ldx=None, ldy=None):
+    v = _capi.CsrView(rows, rows, nnz, BASE, BASE + 4096 + idx_off, BASE + 8192, vt)
+    return _capi.BatchItem(v, k, BASE + 16384, ldx if ldx is not None else (k if k > 1 else 1),
+                           BASE + 32768, ldy if ldy is not None else (k if k > 1 else 1),
+                           _capi.SCHEDULES[schedule], ipt)
+
+
+def query(items):
+    arr = (_capi.BatchItem * len(items))(*items)
+    info = _capi.BatchInfo()
+    st = _capi.lib().loops_batch_query(arr, len(items), ctypes.byref(info))
+    return st, info
+
+
+def test_layout_counts():
+    st, info = query([item(1000, 9000, ipt=8)])
+    assert st == _capi.SUCCESS
+    assert (info.spmv_tiles, info.coordinates, info.work_items) == (5, 6, 5)
+    assert info.max_items_per_thread == 8 and info.spmm_work_items == 0
+    st, info = query([item(1000, 9000, schedule="thread_mapped")])
+    assert st == _capi.SUCCESS and info.work_items == 4 and info.spmv_tiles == 0
+    st, info = query([item(1000, 9000, schedule="group_mapped", ipt=16)])
+    assert st == _capi.SUCCESS and info.work_items == 63
+    st, info = query([item(1000, 9000, k=8)])  # 40 units of 256 steps, 8 lanes per unit
+    assert st == _capi.SUCCESS
+    assert (info.spmm_units, info.spmm_work_items, info.work_items) == (40, 2, 2)
+    st, info = query([item(1000, 9000, ipt=4), item(1000, 9000, k=8), item(50, 0, ipt=16)])
+    assert st == _capi.SUCCESS
+    assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 2
+    assert info.work_items == 13 and info.max_items_per_thread == 16
+    assert info.table_bytes > 0 and info.workspace_bytes > 0
+
+
+def test_layout_rejections():
+    NS, IV = _capi.ERROR_NOT_SUPPORTED, _capi.ERROR_INVALID_VALUE
+    assert query([item(1000, 9000, idx_off=4)])[0] == NS          # unaligned indices
+    assert query([item(1000, 9000, schedule="group_mapped", ipt=3)])[0] == NS
+    assert query([item(1000, 9000, schedule="work_oriented")])[0] == NS
+    assert query([item(1000, 9000, ipt=5)])[0] == NS
+    assert query([item(1000, 9000, k=4, ipt=8)])[0] == NS          # SpMM unit < 16 steps
+    assert query([item(1000, 9000, k=0)])[0] == IV
+    assert query([item(1000, 9000, k=8, ldx=4)])[0] == IV          # ldx < k
+    assert query([item(-1, 0)])[0] == IV
+    assert query([item(1000, 9000, vt=_capi.VALUE_F16)])[0] == _capi.SUCCESS
+    assert query([item(2**30, 2**30)])[0] == _capi.ERROR_OVERFLOW
+
+
+def test_tune_features_and_lookup(tmp_path):
+    assert tune.features(1000, 12000, 400, 1) == "k0/s6/d3/w2"
+    assert tune.features(1000, 12000, 400, 32) == "k5/s6/d3/w2"
+    assert tune.features(0, 0, 0, 1) == "k0/s0/d0/w0"
+    t = tune.Table()
+    assert t.pick(1000, 12000, 400) == tune.DEFAULT_SPMV
+    assert t.pick(1000, 12000, 400, 8) == tune.DEFAULT_SPMM
+    t.data["spmv"]["k0/s6/d3/w2"] = {"schedule": "merge_path_flat", "tile": "16"}
+    t.data["spmv"]["k0/s9/d0/w0"] = {"schedule": "group_mapped", "tile": "x4"}
+    assert t.pick(1000, 12000, 400) == ("merge_path_flat", 16)
+    assert t.pick(1100, 13000, 400) == ("merge_path_flat", 16)     # nearest bucket
+    assert t.pick(200000, 200000, 1) == ("group_mapped", 8)        # 2 lanes x 4
+    p = tmp_path / "t.json"
+    t.save(str(p))
+    assert tune.Table.load(str(p)).pick(1000, 12000, 400) == ("merge_path_flat", 16)
+
+
+def test_committed_table_is_complete():
+    t = tune.Table.load()
+    assert t.data["spmv"] and t.data["spmm"]
+    for sec in ("spmv", "spmm"):
+        for key, e in t.data[sec].items():
+            assert e["schedule"] in _capi.SCHEDULES
+            sched, ipt = t.pick(1000, 5000, 50, 1 if sec == "spmv" else 8)
+            assert sched in _capi.SCHEDULES and ipt >= 0

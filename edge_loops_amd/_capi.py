@@ -113,6 +113,46 @@ class BatchInfo(ctypes.Structure):
     ]
 
 
+# loops_server_mode_t
+SERVER_BATCHED = 0
+SERVER_PLAIN = 1
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("arena_bytes", ctypes.c_int64),
+        ("scratch_bytes", ctypes.c_int64),
+        ("slots", ctypes.c_int32),
+        ("max_batch_requests", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class ServerMatrix(ctypes.Structure):
+    _fields_ = [
+        ("A", CsrView),
+        ("spmv_schedule", ctypes.c_int32),
+        ("spmv_items_per_thread", ctypes.c_int32),
+        ("spmm_items_per_unit", ctypes.c_int32 * 8),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "submitted", "completed", "batches", "batched_requests", "arena_bytes",
+        "arena_high_water")]
+
+
+class ServerRequest(ctypes.Structure):
+    _fields_ = [
+        ("matrix", ctypes.c_int32),
+        ("k", ctypes.c_int64),
+        ("x", ctypes.c_void_p),
+        ("ldx", ctypes.c_int64),
+    ]
+
+
 class LoopsError(RuntimeError):
     """Raised for a non-success loops_status_t (the C++ layer's
     error::exception_t, reference include/loops/error.hxx:21-45)."""
@@ -152,6 +192,22 @@ SIGNATURES = {
     "loops_batch_prepare": (_st, [ctypes.POINTER(BatchItem), _i64, _vp, ctypes.c_size_t, _vp,
                                   ctypes.c_size_t, ctypes.POINTER(BatchInfo)]),
     "loops_batch_launch": (_st, [_vp, ctypes.POINTER(BatchInfo), _i32, _vp]),
+    "loops_server_create": (_st, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
+    "loops_server_destroy": (_st, [_vp]),
+    "loops_server_add_matrix": (_st, [_vp, ctypes.POINTER(ServerMatrix),
+                                      ctypes.POINTER(ctypes.c_int32)]),
+    "loops_server_submit": (_st, [_vp, ctypes.c_int32, _i64, _vp, _i64,
+                                  ctypes.POINTER(ctypes.c_int64)]),
+    "loops_server_test": (_st, [_vp, _i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_vp),
+                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "loops_server_wait": (_st, [_vp, _i64, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]),
+    "loops_server_release": (_st, [_vp, _i64]),
+    "loops_server_now": (_st, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "loops_server_stats": (_st, [_vp, ctypes.POINTER(ServerStats)]),
+    "loops_server_replay": (_st, [_vp, ctypes.POINTER(ServerRequest), _i64,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]),
     "loops_spmm_workspace_size": (_st, [_vp, _st, ctypes.POINTER(CsrView), _i64,
                                         ctypes.POINTER(Params),
                                         ctypes.POINTER(ctypes.c_size_t)]),

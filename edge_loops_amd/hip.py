@@ -28,6 +28,8 @@ def runtime():
                                              ctypes.c_void_p, ctypes.c_void_p]
         _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
         _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                   ctypes.c_int]
     return _hip
 
 
@@ -90,3 +92,8 @@ class timer_t:
 
 def stream_synchronize(stream):
     _ok(runtime().hipStreamSynchronize(stream), "hipStreamSynchronize")
+
+
+def memcpy(dst, src, nbytes):
+    """Synchronous hipMemcpy between device (or host) addresses."""
+    _ok(runtime().hipMemcpy(dst, src, int(nbytes), 4), "hipMemcpy")  # hipMemcpyDefault

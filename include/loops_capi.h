@@ -257,6 +257,86 @@ loops_status_t loops_batch_launch(const void* device_table,
                                   const loops_batch_info_t* info, int32_t grid,
                                   hipStream_t stream);
 
+/* ---- Serving: continuous batching over HBM-resident matrices ----------
+ * No counterpart in the reference.  A server owns a device arena for
+ * request outputs, `slots` streams each with a batch table and workspace of
+ * scratch_bytes, and a thread that admits requests in arrival order once
+ * their output fits in the arena and launches everything admitted as one
+ * batch (loops_batch_*) on a free slot, so batches overlap on the device.
+ * Mode PLAIN launches each request alone through loops_spmv / loops_spmm on
+ * one stream instead (the per-call baseline, same admission and clock).
+ * Results are bit-identical whatever else is in flight. */
+typedef enum loops_server_mode_t {
+  LOOPS_SERVER_BATCHED = 0,
+  LOOPS_SERVER_PLAIN = 1
+} loops_server_mode_t;
+
+typedef struct loops_server_config_t {
+  int64_t arena_bytes;        /* outputs of admitted requests; 0 = 1 GiB        */
+  int64_t scratch_bytes;      /* per slot: batch table + workspace; 0 = 64 MiB  */
+  int32_t slots;              /* batches in flight (BATCHED, 0 = 2) or requests
+                                 in flight (PLAIN, 0 = 64)                      */
+  int32_t max_batch_requests; /* 0 = 512                                        */
+  int32_t mode;               /* loops_server_mode_t                            */
+  uint32_t flags;             /* reserved, 0                                    */
+} loops_server_config_t;
+
+/* A resident matrix (device CSR, owned by the caller, kept valid while the
+ * server lives) and its tunings: the schedule / items per thread of its
+ * SpMV requests (loops_batch_item_t rules) and the merge steps per SpMM unit
+ * by k: [0] k = 2, [1] k <= 4, [2] <= 8, [3] <= 16, [4] <= 32, [5] <= 64,
+ * [6] <= 128, [7] larger (0 = 256). */
+typedef struct loops_server_matrix_t {
+  loops_csr_view_t A;
+  int32_t spmv_schedule;
+  int32_t spmv_items_per_thread;
+  int32_t spmm_items_per_unit[8];
+} loops_server_matrix_t;
+
+typedef struct loops_server_stats_t {
+  int64_t submitted, completed, batches, batched_requests;
+  int64_t arena_bytes, arena_high_water;
+} loops_server_stats_t;
+
+typedef struct loops_server* loops_server_t;
+
+loops_status_t loops_server_create(const loops_server_config_t* config,
+                                   loops_server_t* server);
+loops_status_t loops_server_destroy(loops_server_t server);
+loops_status_t loops_server_add_matrix(loops_server_t server,
+                                       const loops_server_matrix_t* matrix,
+                                       int32_t* id);
+/* Request Y = A X with X (cols x k, leading dimension ldx >= k; a plain
+ * vector when k == 1) in device memory, valid until the request completes.
+ * The output (rows x k, leading dimension k) is allocated by the server. */
+loops_status_t loops_server_submit(loops_server_t server, int32_t matrix, int64_t k,
+                                   const float* x, int64_t ldx, int64_t* ticket);
+/* done = 1 once complete; y then points at the output until release.
+ * Times are seconds on the server's clock (loops_server_now). */
+loops_status_t loops_server_test(loops_server_t server, int64_t ticket, int32_t* done,
+                                 float** y, double* submit_s, double* done_s);
+loops_status_t loops_server_wait(loops_server_t server, int64_t ticket, float** y,
+                                 double* submit_s, double* done_s);
+/* Frees a completed request's output (arena memory for admission). */
+loops_status_t loops_server_release(loops_server_t server, int64_t ticket);
+loops_status_t loops_server_now(loops_server_t server, double* now_s);
+loops_status_t loops_server_stats(loops_server_t server, loops_server_stats_t* stats);
+
+/* Open-loop load: request i is submitted arrival_s[i] seconds (non-decreasing)
+ * after the call starts and released once complete; latency_s[i] =
+ * completion - scheduled arrival; makespan_s = last completion - first
+ * arrival. */
+typedef struct loops_server_request_t {
+  int32_t matrix;
+  int64_t k;
+  const float* x;
+  int64_t ldx;
+} loops_server_request_t;
+loops_status_t loops_server_replay(loops_server_t server,
+                                   const loops_server_request_t* requests, int64_t n,
+                                   const double* arrival_s, double* latency_s,
+                                   double* makespan_s);
+
 /* SpMM, C = A * B with B (cols x k) and C (rows x k) dense row-major fp32
  * (container/matrix.cuh:27-35) with leading dimensions ldb, ldc >= k.
  * Schedules: THREAD_MAPPED (algorithms/spmm/thread_mapped.cuh:28-53),

@@ -118,3 +118,86 @@ def batched(count=512, reps=10, table=None, log=print):
     for k in ("plain_eager", "plain_graph"):
         out[f"speedup_tuned_vs_{k}"] = out[k]["ms_per_pass"] / out["batched_tuned"]["ms_per_pass"]
     return out
+
+
+def _pcts(lat):
+    lat = np.asarray(lat) * 1e3
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean())}
+
+
+def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
+    """Served workload: 64 HBM-resident matrices (synth.c5_pool_64), requests
+    with a uniform random matrix and k uniform in [1, 32] fp32 right-hand
+    sides (k == 1: a vector), seeded Poisson arrivals.  Batched server vs the
+    plain per-call mode on the same loop and clock."""
+    import torch
+    from . import tune
+    from .serve import Server, matrix_tunings, poisson_arrivals
+    t0 = time.time()
+    mats = synth.c5_pool_64()
+    log(f"served: generated 64 matrices in {time.time() - t0:.1f}s")
+    table = table or tune.Table.load()
+    pools = {}
+    for vt in ("f32", "bf16"):
+        dev = [csr_t.from_host(m.rows, m.cols, m.offsets, m.indices, m.values, value_type=vt)
+               for m in mats]
+        tun = [matrix_tunings(A, table, m.max_row()) for A, m in zip(dev, mats)]
+        pools[vt] = (dev, tun)
+    x1 = [torch.rand(m.cols, device="cuda") * 2 - 1 for m in mats]
+    X32 = [torch.rand(m.cols, 32, device="cuda") * 2 - 1 for m in mats]
+    rng = np.random.default_rng(seed)
+    mid = rng.integers(0, len(mats), n_requests)
+    ks = rng.integers(1, 33, n_requests)
+    flops = float(sum(2 * mats[m].nnz * k for m, k in zip(mid, ks)))
+    out = {"workload": "64 CSR matrices, rows log-uniform [1e4, 4e6], nnz log-uniform "
+                       "[1e5, 8e6], Zipf 1.8-2.2; requests: uniform matrix, k uniform "
+                       "[1, 32] fp32 RHS, seeded Poisson arrivals",
+           "matrices": len(mats), "pool_nnz": int(sum(m.nnz for m in mats)),
+           "pool_rows": int(sum(m.rows for m in mats)), "requests": n_requests,
+           "mean_k": float(ks.mean()), "slots": slots}
+
+    def server(mode, vt):
+        srv = Server(mode=mode, arena_bytes=32 << 30, scratch_bytes=1 << 30,
+                     slots=slots if mode == "batched" else 64)
+        dev, tun = pools[vt]
+        ids = [srv.add_matrix(A, tunings=t) for A, t in zip(dev, tun)]
+        reqs = [(ids[m], x1[m] if k == 1 else X32[m][:, :k]) for m, k in zip(mid, ks)]
+        srv.replay(reqs[:200], np.zeros(200))  # warm-up
+        return srv, reqs
+
+    def run(srv, reqs, rate):
+        arr = np.zeros(len(reqs)) if rate is None else poisson_arrivals(len(reqs), rate, seed + 1)
+        lat, span = srv.replay(reqs, arr)
+        r = _pcts(lat)
+        r["requests_per_s"] = len(reqs) / span
+        r["gflops"] = flops / span / 1e9
+        r["offered_per_s"] = rate
+        return r
+
+    res = {}
+    srv, reqs = server("plain", "f32")
+    res["plain_f32_burst"] = run(srv, reqs, None)
+    cap_plain = res["plain_f32_burst"]["requests_per_s"]
+    lam0 = 0.7 * cap_plain
+    res["plain_f32_at_load"] = run(srv, reqs, lam0)
+    srv.close()
+    log(f"plain: capacity {cap_plain:.0f} req/s; at {lam0:.0f}/s "
+        f"p99 {res['plain_f32_at_load']['p99_ms']:.2f} ms")
+    for vt in ("f32", "bf16"):
+        srv, reqs = server("batched", vt)
+        res[f"batched_{vt}_burst"] = b = run(srv, reqs, None)
+        cap = b["requests_per_s"]
+        res[f"batched_{vt}_at_load"] = run(srv, reqs, lam0)
+        hi = min(3.0 * lam0, 0.9 * cap)
+        res[f"batched_{vt}_at_3x_load"] = run(srv, reqs, hi)
+        st = srv.stats()
+        res[f"batched_{vt}_mean_batch"] = st["batched_requests"] / max(st["batches"], 1)
+        srv.close()
+        log(f"batched {vt}: capacity {cap:.0f} req/s; at {lam0:.0f}/s p99 "
+            f"{res[f'batched_{vt}_at_load']['p99_ms']:.2f} ms; at {hi:.0f}/s p99 "
+            f"{res[f'batched_{vt}_at_3x_load']['p99_ms']:.2f} ms")
+    out.update(res)
+    out["capacity_ratio_f32"] = res["batched_f32_burst"]["requests_per_s"] / cap_plain
+    out["offered_load_per_s"] = lam0
+    return out

@@ -11,9 +11,9 @@
 //   GROUP  256/G consecutive rows of an SpMV item, G lanes per row (G = 2..64,
 //          a power of two) with a fixed xor-butterfly reduction, for rows
 //          of moderate, even length;
-//   SPMM   256/G merge-path units of an SpMM item, G lanes spanning its
-//          columns per unit, 1 or 4 columns per lane (the single-matrix SpMM
-//          merge-path unit).
+//   SPMM   one merge-path SpMM tile of an SpMM item (the single-matrix SpMM
+//          tile: ranges of G lanes spanning its columns, 1 or 4 columns per
+//          lane, IPT = 4, 8 or 16 steps per range).
 // The caller copies the table to the device and calls loops_batch_launch:
 // a prologue kernel zeroes the work-queue counters and writes every merge
 // tile's start coordinate (one lane per coordinate), a persistent SpMV
@@ -46,13 +46,14 @@ struct batch_item_dev {
   const float* x;
   float* y;
   fixup_ws fx;  // MERGE: this item's tile slots
-  loops_coord_t* coords;  // MERGE: tiles + 1 start coordinates
-  spmm_ws sx;   // SPMM: this item's unit slots
+  loops_coord_t* coords;  // MERGE, SPMM: tiles + 1 start coordinates
+  spmm_ws sx;   // SPMM: this item's tile slots
   long long ldx, ldy;
   int rows, cols, nnz, k;
-  int vtype, kind, ipt, g;  // ipt: merge steps per lane (MERGE), per unit (SPMM)
-                            // g: lanes per row (GROUP) or per unit (SPMM)
-  int units, first_slot, cpl, first_coord;  // cpl: columns per lane (SPMM)
+  int vtype, kind, ipt, g;  // ipt: merge steps per lane (MERGE) / range (SPMM)
+                            // g: lanes per row (GROUP) or per range (SPMM)
+  int units, first_slot, cpl, first_coord;  // units: SPMM tiles; cpl: columns per lane
+  int ti, pad0, pad1, pad2;                 // ti: merge steps per tile (MERGE, SPMM)
 };
 
 struct batch_work {
@@ -142,13 +143,15 @@ __device__ __forceinline__ void run_group(const batch_item_dev& it, int block) {
   if (row < it.rows && lane == 0) it.y[row] = sum;
 }
 
-template <int G, int CPL, typename V>
-__device__ __forceinline__ void run_spmm(const batch_item_dev& it, int first) {
-  const long long u = first + (long long)(threadIdx.x / G);
-  if (u >= it.units) return;  // uniform within the group
-  spmm_merge_unit<G, CPL, V>(it.offsets, it.indices, static_cast<const V*>(it.values),
-                           it.rows, it.nnz, it.k, it.x, it.ldx, it.y, it.ldy,
-                           it.ipt, u, it.sx);
+using batch_spmm_smem = spmm_smem<16, 4>;  // the largest SpMM tile's LDS
+
+template <int IPT, int CPL, typename V>
+__device__ __forceinline__ void run_spmm(const batch_item_dev& it, int tile,
+                                         batch_spmm_smem& raw) {
+  auto& sm = *reinterpret_cast<spmm_smem<IPT, CPL>*>(&raw);
+  spmm_tile<IPT, CPL, V>(it.offsets, it.indices, static_cast<const V*>(it.values), it.rows,
+                         it.nnz, it.k, it.x, it.ldx, it.y, it.ldy, it.g, tile, it.coords,
+                         it.sx, sm);
 }
 
 template <typename V, int MAXIPT>
@@ -182,31 +185,22 @@ __device__ __forceinline__ void run_spmv_piece(const batch_item_dev& it,
 }
 
 template <typename V>
-__device__ __forceinline__ void run_spmm_piece(const batch_item_dev& it, const batch_work& w) {
+__device__ __forceinline__ void run_spmm_piece(const batch_item_dev& it, const batch_work& w,
+                                               batch_spmm_smem& sm) {
   if (it.cpl == 4) {
-    switch (it.g) {
-      case 4: run_spmm<4, 4, V>(it, w.sub); break;
-      case 8: run_spmm<8, 4, V>(it, w.sub); break;
-      case 16: run_spmm<16, 4, V>(it, w.sub); break;
-      case 32: run_spmm<32, 4, V>(it, w.sub); break;
-      default: run_spmm<64, 4, V>(it, w.sub); break;
-    }
+    if (it.ipt == 4) run_spmm<4, 4, V>(it, w.sub, sm);
+    else if (it.ipt == 16) run_spmm<16, 4, V>(it, w.sub, sm);
+    else run_spmm<8, 4, V>(it, w.sub, sm);
   } else {
-    switch (it.g) {
-      case 1: run_spmm<1, 1, V>(it, w.sub); break;
-      case 2: run_spmm<2, 1, V>(it, w.sub); break;
-      case 4: run_spmm<4, 1, V>(it, w.sub); break;
-      case 8: run_spmm<8, 1, V>(it, w.sub); break;
-      case 16: run_spmm<16, 1, V>(it, w.sub); break;
-      case 32: run_spmm<32, 1, V>(it, w.sub); break;
-      default: run_spmm<64, 1, V>(it, w.sub); break;
-    }
+    if (it.ipt == 4) run_spmm<4, 1, V>(it, w.sub, sm);
+    else if (it.ipt == 16) run_spmm<16, 1, V>(it, w.sub, sm);
+    else run_spmm<8, 1, V>(it, w.sub, sm);
   }
 }
 
 // Prologue: zero the work-queue counters; write the start coordinate of
-// every merge tile of every MERGE item (and the end of its last tile):
-// coordinate c belongs to item coord_item[c].
+// every merge tile of every MERGE and SPMM item (and the end of its last
+// tile): coordinate c belongs to item coord_item[c].
 __global__ __launch_bounds__(256) void batch_prologue_kernel(const char* __restrict__ table) {
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
@@ -218,7 +212,7 @@ __global__ __launch_bounds__(256) void batch_prologue_kernel(const char* __restr
   const batch_item_dev& it = items[coord_item[c]];
   const long long t = c - it.first_coord;
   const long long total = (long long)it.rows + it.nnz;
-  long long d = t * (long long)kBatchTpb * it.ipt;
+  long long d = t * (long long)it.ti;
   if (d > total) d = total;
   const auto xy = loops::search::_binary_search<long long>(
       d, it.offsets + 1, 0LL, (long long)it.rows, (long long)it.nnz);
@@ -255,6 +249,7 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmv_kernel(const char* __res
 }
 
 __global__ __launch_bounds__(kBatchTpb) void batch_spmm_kernel(const char* __restrict__ table) {
+  __shared__ batch_spmm_smem sm;
   __shared__ long long s_next;
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
@@ -268,9 +263,9 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmm_kernel(const char* __res
     if (threadIdx.x == 0) claim = pq.claim();
     const batch_work w = work[h->n_spmv_work + i];
     const batch_item_dev& it = items[w.item];
-    if (it.vtype == LOOPS_VALUE_BF16) run_spmm_piece<bf16_t>(it, w);
-    else if (it.vtype == LOOPS_VALUE_F16) run_spmm_piece<f16_t>(it, w);
-    else run_spmm_piece<float>(it, w);
+    if (it.vtype == LOOPS_VALUE_BF16) run_spmm_piece<bf16_t>(it, w, sm);
+    else if (it.vtype == LOOPS_VALUE_F16) run_spmm_piece<f16_t>(it, w, sm);
+    else run_spmm_piece<float>(it, w, sm);
     if (threadIdx.x == 0) s_next = claim;
     __syncthreads();
     i = s_next;
@@ -311,7 +306,7 @@ __global__ __launch_bounds__(256) void batch_spmv_fixup_kernel(const char* __res
   }
 }
 
-// SpMM unit fix-up across the batch: one workgroup per global unit slot.
+// SpMM tile fix-up across the batch: one workgroup per global tile slot.
 __global__ __launch_bounds__(64) void batch_spmm_fixup_kernel(const char* __restrict__ table) {
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
@@ -321,7 +316,7 @@ __global__ __launch_bounds__(64) void batch_spmm_fixup_kernel(const char* __rest
   if (s >= h->n_units) return;
   const int2 iu = unit_item[s];
   const batch_item_dev& it = items[iu.x];
-  spmm_fixup_unit(it.offsets, it.rows, it.k, it.ipt, iu.y, it.sx, it.y, it.ldy);
+  spmm_fixup_unit(it.offsets, it.rows, it.k, it.ti, iu.y, it.sx, it.y, it.ldy);
 }
 
 // ---------------------------------------------------------------------------
@@ -359,7 +354,8 @@ inline spmm_pick spmm_shape_for(const loops_batch_item_t& b) {
     while (g < 64 && g * 4 < b.k) g *= 2;
     return {g, 4};
   }
-  return {next_pow2_le64(b.k), 1};
+  const int g = next_pow2_le64(b.k);
+  return {g < kSpmmMinG ? kSpmmMinG : g, 1};
 }
 }  // namespace
 
@@ -408,12 +404,13 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
       if (ipt > max_ipt) max_ipt = ipt;
     } else {
       if (b.schedule != LOOPS_SCHEDULE_MERGE_PATH_FLAT) return LOOPS_ERROR_NOT_SUPPORTED;
-      const long long ipu = b.items_per_thread ? b.items_per_thread : 256;
-      if (ipu < 16 || ipu > 65536) return LOOPS_ERROR_NOT_SUPPORTED;
-      const long long U = (total + ipu - 1) / ipu;
-      const int g = spmm_shape_for(b).g;
-      n_spmm_work += (U + kBatchTpb / g - 1) / (kBatchTpb / g);
+      const int ipt = b.items_per_thread ? b.items_per_thread : 8;
+      if (ipt != 4 && ipt != 8 && ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
+      const long long ti = spmm_tile_items(spmm_shape_for(b).g, ipt);
+      const long long U = (total + ti - 1) / ti;
+      n_spmm_work += U;
       n_units += U;
+      n_coords += U + 1;
       spmm_ws_bytes += U > 0 ? a256(spmm_ws_bytes_impl(U, (int)b.k)) : 0;
     }
   }
@@ -487,7 +484,8 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
     } else if (b.k == 1) {
       d.kind = BK_MERGE;
       d.ipt = b.items_per_thread ? b.items_per_thread : 8;
-      const long long T = (total + (long long)kBatchTpb * d.ipt - 1) / ((long long)kBatchTpb * d.ipt);
+      d.ti = kBatchTpb * d.ipt;
+      const long long T = (total + d.ti - 1) / d.ti;
       d.first_slot = (int)slot;
       d.fx.carry_row = all_tiles.carry_row + slot;
       d.fx.seg_end = all_tiles.seg_end + slot;
@@ -504,18 +502,22 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
       coord += T + 1;
     } else {
       d.kind = BK_SPMM;
-      d.ipt = b.items_per_thread ? b.items_per_thread : 256;
+      d.ipt = b.items_per_thread ? b.items_per_thread : 8;
       const spmm_pick sp = spmm_shape_for(b);
       d.g = sp.g;
       d.cpl = sp.cpl;
-      const long long U = (total + d.ipt - 1) / d.ipt;
+      d.ti = (int)spmm_tile_items(d.g, d.ipt);
+      const long long U = (total + d.ti - 1) / d.ti;
       d.units = (int)U;
       if (U > 0) {
         d.sx = carve_spmm_ws(spmm_base, U, (int)b.k);
         spmm_base += a256(spmm_ws_bytes_impl(U, (int)b.k));
       }
-      const int per = kBatchTpb / d.g;
-      for (long long q = 0; q < U; q += per) work[wm++] = batch_work{(int)i, BK_SPMM, (int)q, 0};
+      d.first_coord = (int)coord;
+      d.coords = all_coords + coord;
+      for (long long t = 0; t <= U; ++t) coord_item[coord + t] = (int)i;
+      coord += U + 1;
+      for (long long q = 0; q < U; ++q) work[wm++] = batch_work{(int)i, BK_SPMM, (int)q, 0};
       for (long long q = 0; q < U; ++q) {
         unit_item[2 * (unit + q)] = (int)i;
         unit_item[2 * (unit + q) + 1] = (int)q;

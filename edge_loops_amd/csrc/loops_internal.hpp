@@ -127,9 +127,9 @@ bool gm_spmm_supported(int tpb);
 hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ldb,
                           float* C, long long ldc, int tpb, hipEvent_t ev0,
                           hipEvent_t ev1, hipStream_t s);
-bool mp_spmm_supported(int tpb, int ipw);
-long long mp_spmm_units(const csr_args& A, int tpb, int ipw);
-size_t mp_spmm_ws_bytes(long long units, int k);
+bool mp_spmm_supported(int tpb, int ipt);
+long long mp_spmm_tiles(const csr_args& A, int k, int ipt);
+size_t mp_spmm_ws_bytes(long long tiles, int k);
 hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ldb,
                           float* C, long long ldc, int tpb, int ipw, void* ws,
                           hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);

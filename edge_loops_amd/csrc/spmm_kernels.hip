@@ -110,12 +110,12 @@ __global__ __launch_bounds__(TPB) void spmm_group_mapped_kernel(
 }
 
 __global__ void spmm_fixup_kernel(const int* __restrict__ offsets, int rows,
-                                  int k, long long ipu, long long units,
+                                  int k, long long ti, long long tiles,
                                   spmm_ws ws, float* __restrict__ C,
                                   long long ldc) {
   const long long t = blockIdx.x;
-  if (t >= units) return;
-  spmm_fixup_unit(offsets, rows, k, ipu, t, ws, C, ldc);
+  if (t >= tiles) return;
+  spmm_fixup_unit(offsets, rows, k, ti, t, ws, C, ldc);
 }
 
 // ---------------------------------------------------------------------------
@@ -172,24 +172,24 @@ hipError_t gm_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
   return hipGetLastError();
 }
 
-template <int TPB, int G, int CPL, typename V>
+template <int IPT, int CPL, typename V>
 hipError_t mp_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
-                     float* C, long long ldc, long long ipu, void* ws,
-                     hipStream_t s, hipEvent_t ev1) {
+                     float* C, long long ldc, int g, void* ws, hipStream_t s,
+                     hipEvent_t ev1) {
   const long long total = (long long)A.rows + A.nnz;
-  const long long units = (total + ipu - 1) / ipu;
-  spmm_ws w = carve_spmm_ws(ws, units, k);
-  long long grid = (units * G + TPB - 1) / TPB;
-  spmm_merge_path_kernel<TPB, G, CPL, V><<<dim3((unsigned)grid), dim3(TPB), 0, s>>>(
-      A.offsets, A.indices, static_cast<const V*>(A.values), A.rows, A.nnz, k,
-      B, ldb, C, ldc, ipu, units, w);
+  const long long ti = spmm_tile_items(g, IPT);
+  const long long tiles = (total + ti - 1) / ti;
+  spmm_ws w = carve_spmm_ws(ws, tiles, k);
+  spmm_merge_path_kernel<IPT, CPL, V><<<dim3((unsigned)tiles), dim3(kSpmmTpb), 0, s>>>(
+      A.offsets, A.indices, static_cast<const V*>(A.values), A.rows, A.nnz, k, B, ldb,
+      C, ldc, g, w);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = record(ev1, s);
   if (e != hipSuccess) return e;
-  if (units > 1) {
-    spmm_fixup_kernel<<<dim3((unsigned)units), dim3(64), 0, s>>>(
-        A.offsets, A.rows, k, ipu, units, w, C, ldc);
+  if (tiles > 1) {
+    spmm_fixup_kernel<<<dim3((unsigned)tiles), dim3(64), 0, s>>>(
+        A.offsets, A.rows, k, ti, tiles, w, C, ldc);
     e = hipGetLastError();
   }
   return e;
@@ -211,12 +211,12 @@ hipError_t gm_spmm_v(const csr_args& A, int k, const float* B, long long ldb,
 
 template <typename V>
 hipError_t mp_spmm_v(const csr_args& A, int k, const float* B, long long ldb,
-                     float* C, long long ldc, spmm_shape sh, long long ipu,
-                     void* ws, hipStream_t s, hipEvent_t ev1) {
-#define X(G_, C_)                  \
-  if (sh.g == G_ && sh.cpl == C_) \
-    return mp_spmm_t<256, G_, C_, V>(A, k, B, ldb, C, ldc, ipu, ws, s, ev1);
-  LOOPS_SPMM_SHAPES(X)
+                     float* C, long long ldc, spmm_shape sh, int ipt, void* ws,
+                     hipStream_t s, hipEvent_t ev1) {
+#define X(I_, C_)                 \
+  if (ipt == I_ && sh.cpl == C_) \
+    return mp_spmm_t<I_, C_, V>(A, k, B, ldb, C, ldc, sh.g, ws, s, ev1);
+  X(4, 1) X(8, 1) X(16, 1) X(4, 4) X(8, 4) X(16, 4)
 #undef X
   return hipErrorInvalidValue;
 }
@@ -226,7 +226,14 @@ inline bool vec4_ok(const float* B, long long ldb, const float* C, long long ldc
          (reinterpret_cast<uintptr_t>(C) % 16 == 0) && (ldc % 4 == 0);
 }
 
-constexpr long long kSpmmDefaultIpu = 256;
+constexpr int kSpmmDefaultIpt = 8;
+
+// Lanes per range of the merge-path SpMM tile (at least kSpmmMinG).
+spmm_shape tile_shape(int k, bool vec4) {
+  spmm_shape sh = pick_shape(k, vec4);
+  if (sh.g < kSpmmMinG) sh.g = kSpmmMinG;
+  return sh;
+}
 
 }  // namespace
 
@@ -261,33 +268,35 @@ hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ld
   return record(ev1, s);
 }
 
-bool mp_spmm_supported(int tpb, int ipu) {
-  return tpb == 256 && (ipu == 0 || (ipu >= 16 && ipu <= 65536));
+bool mp_spmm_supported(int tpb, int ipt) {
+  return tpb == 256 && (ipt == 0 || ipt == 4 || ipt == 8 || ipt == 16);
 }
 
-long long mp_spmm_units(const csr_args& A, int tpb, int ipu) {
-  (void)tpb;
-  long long per = ipu ? ipu : kSpmmDefaultIpu;
-  return ((long long)A.rows + A.nnz + per - 1) / per;
+// Tiles for the 1-column-per-lane shape, which has the most: an upper bound
+// for any B / C alignment.
+long long mp_spmm_tiles(const csr_args& A, int k, int ipt) {
+  const spmm_shape sh = tile_shape(k, false);
+  const long long ti = spmm_tile_items(sh.g, ipt ? ipt : kSpmmDefaultIpt);
+  return ((long long)A.rows + A.nnz + ti - 1) / ti;
 }
 
-size_t mp_spmm_ws_bytes(long long units, int k) {
-  return units > 0 && k > 0 ? spmm_ws_bytes_impl(units, k) : 0;
+size_t mp_spmm_ws_bytes(long long tiles, int k) {
+  return tiles > 0 && k > 0 ? spmm_ws_bytes_impl(tiles, k) : 0;
 }
 
 hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ldb,
-                          float* C, long long ldc, int tpb, int ipu, void* ws,
+                          float* C, long long ldc, int tpb, int ipt, void* ws,
                           hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
   (void)tpb;
   hipError_t e = record(ev0, s);
   if (e != hipSuccess) return e;
   if ((long long)A.rows + A.nnz == 0 || k == 0 || A.rows == 0) return record(ev1, s);
-  const long long per = ipu ? ipu : kSpmmDefaultIpu;
-  spmm_shape sh = pick_shape(k, vec4_ok(B, ldb, C, ldc));
+  const int it = ipt ? ipt : kSpmmDefaultIpt;
+  const spmm_shape sh = tile_shape(k, vec4_ok(B, ldb, C, ldc));
   switch (A.vtype) {
-    case LOOPS_VALUE_F32: return mp_spmm_v<float>(A, k, B, ldb, C, ldc, sh, per, ws, s, ev1);
-    case LOOPS_VALUE_BF16: return mp_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, per, ws, s, ev1);
-    case LOOPS_VALUE_F16: return mp_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, per, ws, s, ev1);
+    case LOOPS_VALUE_F32: return mp_spmm_v<float>(A, k, B, ldb, C, ldc, sh, it, ws, s, ev1);
+    case LOOPS_VALUE_BF16: return mp_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, it, ws, s, ev1);
+    case LOOPS_VALUE_F16: return mp_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, it, ws, s, ev1);
   }
   return hipErrorInvalidValue;
 }

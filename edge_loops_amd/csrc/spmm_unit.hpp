@@ -1,6 +1,6 @@
 // SpMM building blocks shared by the single-matrix kernels and the batch
-// kernel: column slices of B and C per lane, the merge-path unit walked by a
-// group of lanes spanning the output columns, and its fixed-order fix-up.
+// kernel: column slices of B and C per lane, the merge-path SpMM tile and
+// its fixed-order fix-up.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,8 +55,8 @@ __device__ __forceinline__ void store_c(float* __restrict__ p, int c, int k,
 }
 
 // ---------------------------------------------------------------------------
-// merge_path: group u walks merge diagonals [u*IPU, (u+1)*IPU).
-// Work space per group: carry_row (int), carry[k] and head[k] (fp32).
+// Fix-up slots of the SpMM tiles: per tile the open row (int) and its carry
+// and head partial sums (k fp32 each).
 // ---------------------------------------------------------------------------
 struct spmm_ws {
   int* carry_row;
@@ -81,112 +81,269 @@ inline spmm_ws carve_spmm_ws(void* ws, long long units, int k) {
   return w;
 }
 
-// One merge-path unit u (merge diagonals [u*ipu, (u+1)*ipu)) walked by the
-// calling group of G lanes (all G lanes call with the same u).
-template <int G, int CPL, typename V>
-__device__ __forceinline__ void spmm_merge_unit(
+// ---------------------------------------------------------------------------
+// Merge-path SpMM tile, C = A * B for k columns.
+//
+// A workgroup of TPB lanes takes one merge tile.  Its lanes form R = TPB / G
+// ranges of G lanes (G = 2 .. 64, a power of two); the G lanes of a range
+// span G * CPL consecutive columns of B and C, and the range walks IPT merge
+// steps, so a tile covers TI = R * IPT steps.
+//  1. Tile end points: read from a precomputed partition, or found by two
+//     wavefronts with the 64-way cooperative search.
+//  2. Row ends, column ids and values of the tile go to LDS (coalesced).
+//  3. Each range finds its start and end in LDS, then issues the B-row loads
+//     of all its nonzeros at once (one round trip) and walks its steps,
+//     storing each row it closes except the first, which it holds back.
+//  4. A segmented scan over the ranges (per column) carries partial sums
+//     between ranges; the held-back rows are completed with it.
+//  5. The row entering the tile open goes to head[tile], the row leaving it
+//     open to carry[tile]; spmm_fixup_unit sums them in tile order.
+// Column blocks of G * CPL columns are walked one after another when k is
+// wider.  Every sum has an order fixed by the matrix, k, G, CPL and IPT.
+// ---------------------------------------------------------------------------
+constexpr int kSpmmTpb = 256;
+constexpr int kSpmmMinG = 2;
+
+template <int IPT, int CPL>
+struct spmm_smem {
+  static constexpr int TI_MAX = (kSpmmTpb / kSpmmMinG) * IPT;
+  static constexpr int NW = kSpmmTpb / 64;
+  int rowend[TI_MAX + 1];
+  int col[TI_MAX];
+  float val[TI_MAX];
+  int coord[5];
+  float wv[NW][64 * CPL];
+  int wf[NW][64];
+};
+
+// Merge steps per tile for lane-group size g.
+__host__ __device__ constexpr long long spmm_tile_items(int g, int ipt) {
+  return (long long)(kSpmmTpb / g) * ipt;
+}
+
+template <int IPT, int CPL, typename V>
+__device__ __forceinline__ void spmm_tile(
     const int* __restrict__ offsets, const int* __restrict__ indices,
     const V* __restrict__ values, int rows, int nnz, int k,
     const float* __restrict__ B, long long ldb, float* __restrict__ C,
-    long long ldc, long long ipu, long long u, const spmm_ws& ws) {
-  constexpr int U = 8;
-  const int lane = threadIdx.x & (G - 1);
+    long long ldc, int G, int tile, const loops_coord_t* __restrict__ coords,
+    const spmm_ws& ws, spmm_smem<IPT, CPL>& sm) {
+  constexpr int TPB = kSpmmTpb;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int TI = (TPB / G) * IPT;
   const long long total = (long long)rows + nnz;
-  long long d0 = u * ipu, d1 = d0 + ipu;
+  long long d0 = (long long)tile * TI;
+  if (d0 > total) d0 = total;
+  long long d1 = d0 + TI;
   if (d1 > total) d1 = total;
-  const auto st = loops::search::_binary_search<long long>(
-      d0, offsets + 1, 0LL, (long long)rows, (long long)nnz);
-  const auto en = loops::search::_binary_search<long long>(
-      d1, offsets + 1, 0LL, (long long)rows, (long long)nnz);
-  const int xs = (int)st.x, ys = (int)st.y, xe = (int)en.x, ye = (int)en.y;
-  const bool open_left = xs < xe && offsets[xs] < ys;
 
-  for (int cb = 0; cb < k; cb += G * CPL) {
-    const int c = cb + lane * CPL;
-    int x = xs, y = ys;
-    // Row ends of rows [rbase, rbase + G) held one per lane.
-    int rbase = xs;
-    int re_reg = offsets[(xs + 1 + lane) < rows ? (xs + 1 + lane) : rows];
-    int row_end = __shfl(re_reg, 0, G);
-    bool first = true;
-    float acc[CPL];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
-
-    auto emit = [&]() {
-      if (first && open_left) {
-        if (c < k) store_c<CPL>(ws.head + u * k + c, c, k, acc);
-      } else if (c < k) {
-        store_c<CPL>(C + (long long)x * ldc + c, c, k, acc);
-      }
-      first = false;
-#pragma unroll
-      for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
-      ++x;
-      if (x - rbase >= G) {
-        rbase = x;
-        int r = x + 1 + lane;
-        re_reg = offsets[r < rows ? r : rows];
-      }
-      row_end = __shfl(re_reg, x - rbase, G);
-    };
-
-    while (y < ye) {
-      const int n = ye - y < U ? ye - y : U;
-      int cj[U];
-      float vj[U];
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        cj[j] = j < n ? indices[y + j] : 0;
-        vj[j] = j < n ? value_traits<V>::load1(values + y + j) : 0.f;
-      }
-      cols_t<CPL> bj[U];
-#pragma unroll
-      for (int j = 0; j < U; ++j)
-        if (j < n) bj[j] = load_b<CPL>(B, ldb, cj[j], c, k);
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        if (j < n) {
-          while (x < xe && row_end <= y) emit();
-#pragma unroll
-          for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
-          ++y;
-        }
+  // 1. tile end points
+  if (coords != nullptr) {
+    if (tid < 2) {
+      const loops_coord_t c = coords[tile + tid];
+      sm.coord[2 * tid] = (int)c.x;
+      sm.coord[2 * tid + 1] = (int)c.y;
+    }
+  } else {
+    if (wave == 0) {
+      auto c = loops::search::wave_search(offsets, rows, nnz, d0);
+      if (lane == 0) {
+        sm.coord[0] = (int)c.x;
+        sm.coord[1] = (int)c.y;
       }
     }
-    while (x < xe) emit();
-    // acc now holds this group's part of row xe (if any).
-    if (c < k) store_c<CPL>(ws.carry + u * k + c, c, k, acc);
+    if (wave == 1) {
+      auto c = loops::search::wave_search(offsets, rows, nnz, d1);
+      if (lane == 0) {
+        sm.coord[2] = (int)c.x;
+        sm.coord[3] = (int)c.y;
+      }
+    }
   }
-  if (lane == 0)
-    ws.carry_row[u] = (xe < rows && offsets[xe] < ye) ? xe : -1;
+  __syncthreads();
+  const int xs = sm.coord[0], ys = sm.coord[1];
+  const int xe = sm.coord[2], ye = sm.coord[3];
+  const int tile_rows = xe - xs;
+  const int tile_nnz = ye - ys;
+  const int tile_items = (int)(d1 - d0);
+
+  // 2. stage the tile: column ids and values, then row ends
+  for (int i = tid; i < tile_nnz; i += TPB) {
+    sm.col[i] = ld_idx(indices + ys + i);
+    sm.val[i] = value_traits<V>::load1(values + ys + i);
+  }
+  for (int i = tid; i <= tile_rows; i += TPB) {
+    int r = xs + 1 + i;
+    if (r > rows) r = rows;
+    sm.rowend[i] = offsets[r] - ys;
+  }
+  if (tid == 0) sm.coord[4] = offsets[xs] - ys;
+  __syncthreads();
+
+  // 3a. this range's start and end in the tile
+  const int r = tid / G;
+  const int g = tid & (G - 1);
+  const int RW = 64 / G;         // ranges per wavefront
+  const int rw = lane / G;       // this range's place in its wavefront
+  auto lds_search = [&](int dd, int* xo, int* yo) {
+    int x_min = dd - tile_nnz;
+    x_min = x_min < 0 ? 0 : x_min;
+    const int x_max = dd < tile_rows ? dd : tile_rows;
+    int first = x_min, count = x_max - x_min;
+    while (count > 0) {
+      const int step = count >> 1;
+      const int idx = first + step;
+      if (sm.rowend[idx] <= dd - idx - 1) {
+        first = idx + 1;
+        count -= step + 1;
+      } else {
+        count = step;
+      }
+    }
+    *xo = first;
+    *yo = dd - first;
+  };
+  int dl = r * IPT;
+  dl = dl < tile_items ? dl : tile_items;
+  int dr = r * IPT + IPT;
+  dr = dr < tile_items ? dr : tile_items;
+  int xl, yl, xl2, yl2;
+  lds_search(dl, &xl, &yl);
+  lds_search(dr, &xl2, &yl2);
+  const int cnt = yl2 - yl;  // nonzeros this range consumes (<= IPT)
+  const bool open_left = tile_rows > 0 && sm.coord[4] < 0;
+  const int xe_start = tile_rows > 0 ? sm.rowend[tile_rows - 1] : sm.coord[4];
+  const bool open_right = xe < rows && xe_start < tile_nnz;
+
+  for (int cb = 0; cb < k; cb += G * CPL) {
+    const int c = cb + g * CPL;
+    // 3b. B rows of this range's nonzeros, all loads in flight together
+    cols_t<CPL> bj[IPT];
+    float vj[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (j < cnt) {
+        vj[j] = sm.val[yl + j];
+        bj[j] = load_b<CPL>(B, ldb, sm.col[yl + j], c, k);
+      }
+    }
+    // 3c. walk
+    float acc[CPL], first_acc[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) acc[q] = first_acc[q] = 0.f;
+    int x = xl, first_row = -1;
+    int row_end = sm.rowend[x];
+    auto close_row = [&]() {
+      if (first_row < 0) {
+        first_row = x;
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) first_acc[q] = acc[q];
+      } else if (c < k) {
+        store_c<CPL>(C + (long long)(xs + x) * ldc + c, c, k, acc);
+      }
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) acc[q] = 0.f;
+      ++x;
+      row_end = sm.rowend[x];
+    };
+    int yy = yl;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (j < cnt) {
+        while (row_end <= yy) close_row();
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
+        ++yy;
+      }
+    }
+    while (x < xl2) close_row();
+
+    // 4. segmented scan over the ranges, per column
+    int f = first_row >= 0 ? 1 : 0;
+    float v[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) v[q] = acc[q];
+    for (int d = 1; d < RW; d <<= 1) {
+      float vu[CPL];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) vu[q] = __shfl_up(v[q], d * G, 64);
+      const int fu = __shfl_up(f, d * G, 64);
+      if (rw >= d) {
+        if (!f) {
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) v[q] = vu[q] + v[q];
+        }
+        f |= fu;
+      }
+    }
+    if (rw == RW - 1) {
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) sm.wv[wave][g * CPL + q] = v[q];
+      sm.wf[wave][g] = f;
+    }
+    __syncthreads();
+    float pv[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) pv[q] = 0.f;
+    for (int w = 0; w < wave; ++w) {
+      const int wf = sm.wf[w][g];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) pv[q] = wf ? sm.wv[w][g * CPL + q] : pv[q] + sm.wv[w][g * CPL + q];
+    }
+    float ev[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) ev[q] = __shfl_up(v[q], G, 64);
+    const int ef = __shfl_up(f, G, 64);
+    if (first_row >= 0 && c < k) {
+      float val[CPL];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q)
+        val[q] = (rw == 0 ? pv[q] : (ef ? ev[q] : pv[q] + ev[q])) + first_acc[q];
+      if (first_row == 0 && open_left) {
+        store_c<CPL>(ws.head + (long long)tile * k + c, c, k, val);
+      } else {
+        store_c<CPL>(C + (long long)(xs + first_row) * ldc + c, c, k, val);
+      }
+    }
+    // 5. the row leaving the tile open: the last range's inclusive value
+    if (r == TPB / G - 1) {
+      float tot[CPL];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) tot[q] = f ? v[q] : pv[q] + v[q];
+      if (c < k) store_c<CPL>(ws.carry + (long long)tile * k + c, c, k, tot);
+      if (cb == 0 && g == 0) ws.carry_row[tile] = open_right ? xe : -1;
+    }
+    __syncthreads();  // sm.wv / sm.wf are reused by the next column block
+  }
 }
 
-template <int TPB, int G, int CPL, typename V>
-__global__ __launch_bounds__(TPB) void spmm_merge_path_kernel(
+template <int IPT, int CPL, typename V>
+__global__ __launch_bounds__(kSpmmTpb) void spmm_merge_path_kernel(
     const int* __restrict__ offsets, const int* __restrict__ indices,
     const V* __restrict__ values, int rows, int nnz, int k,
     const float* __restrict__ B, long long ldb, float* __restrict__ C,
-    long long ldc, long long ipu, long long units, spmm_ws ws) {
-  const long long u = ((long long)blockIdx.x * TPB + threadIdx.x) / G;
-  if (u >= units) return;  // uniform within the group
-  spmm_merge_unit<G, CPL, V>(offsets, indices, values, rows, nnz, k, B, ldb, C,
-                             ldc, ipu, u, ws);
+    long long ldc, int G, spmm_ws ws) {
+  __shared__ spmm_smem<IPT, CPL> sm;
+  spmm_tile<IPT, CPL, V>(offsets, indices, values, rows, nnz, k, B, ldb, C, ldc, G,
+                         blockIdx.x, nullptr, ws, sm);
 }
 
-// Per (unit, column): the unit where an open row starts sums the unit
-// partials of that row in unit order (see spmv_fixup_kernel).
-// Called by a whole workgroup for unit t; its lanes stride over the columns.
+// Per (tile, column): the tile where an open row starts sums the tile
+// partials of that row in tile order (see spmv_fixup_kernel); ti = merge
+// steps per tile.  Called by a whole workgroup for tile t; its lanes stride
+// over the columns.
 __device__ __forceinline__ void spmm_fixup_unit(const int* __restrict__ offsets,
-                                                int rows, int k, long long ipu,
+                                                int rows, int k, long long ti,
                                                 long long t, const spmm_ws& ws,
                                                 float* __restrict__ C,
                                                 long long ldc) {
   const int r = ws.carry_row[t];
   if (r < 0 || r >= rows) return;
   const long long fr = (long long)offsets[r] + r;
-  if (fr / ipu != t) return;
-  const long long t_end = ((long long)offsets[r + 1] + r + 1 - 1) / ipu;
+  if (fr / ti != t) return;
+  const long long t_end = ((long long)offsets[r + 1] + r + 1 - 1) / ti;
   for (int c = threadIdx.x; c < k; c += blockDim.x) {
     float s = ws.carry[t * k + c];
     for (long long v = t + 1; v < t_end; ++v) s += ws.carry[v * k + c];

@@ -8,14 +8,14 @@ group-mapped).  Here the candidates are the batch kernel's piece kinds:
   k == 1: merge_path_flat with 4, 8 or 16 steps per lane; thread_mapped;
           group_mapped with G lanes per row (G from the mean row length, and
           4x that);
-  k >  1: merge_path_flat SpMM units of 64 .. 1024 merge steps.
+  k >  1: merge_path_flat SpMM tiles of 4, 8 or 16 merge steps per range.
 
 For each bucket the tuning matrices that fall in it are run as one batch per
 candidate (items repeated until the batch holds enough work to fill the
 GPU), captured in a graph and replayed; the fastest candidate wins.  The
 table is plain JSON: bucket key -> {"schedule", "tile", "us": {candidate:
 microseconds}}.  Lookup falls back to the nearest tuned bucket, then to
-merge_path_flat with 8 steps per lane (SpMM: 256 steps per unit).
+merge_path_flat with 8 steps per lane (SpMM: 8 steps per range).
 """
 from __future__ import annotations
 
@@ -26,9 +26,9 @@ import os
 import numpy as np
 
 SPMV_MERGE = [4, 8, 16]
-SPMM_IPU = [64, 128, 256, 512, 1024]
+SPMM_IPT = [4, 8, 16]
 DEFAULT_SPMV = ("merge_path_flat", 8)
-DEFAULT_SPMM = ("merge_path_flat", 256)
+DEFAULT_SPMM = ("merge_path_flat", 8)
 TABLE_PATH = os.path.join(os.path.dirname(__file__), "tuning", "mi355x_batch.json")
 
 
@@ -75,7 +75,7 @@ def resolve(sched, tile, rows, nnz):
 
 
 def spmm_candidates():
-    return [("merge_path_flat", u) for u in SPMM_IPU]
+    return [("merge_path_flat", u) for u in SPMM_IPT]
 
 
 class Table:
@@ -114,7 +114,10 @@ class Table:
             return DEFAULT_SPMV if k == 1 else DEFAULT_SPMM
         # JSON keeps tiles as strings ("8", "x4"); numeric ones become int.
         t = e["tile"]
-        return e["schedule"], int(t) if str(t).isdigit() else t
+        t = int(t) if str(t).isdigit() else t
+        if k > 1 and t not in SPMM_IPT:
+            return DEFAULT_SPMM  # an entry from another tile design
+        return e["schedule"], t
 
 
 def tuning_set(count=192, seed=77):
@@ -207,15 +210,15 @@ def tune(mats, ks=(2, 4, 8, 16, 32), min_work=30_000_000, log=print):
             rep = min(max(1, math.ceil(min_work * 4 / max(work, 1))), 32)
             times = {}
             X = {}
-            for sched, ipu in spmm_candidates():
+            for sched, ipt in spmm_candidates():
                 items = []
                 for r in range(rep):
                     for m, A in group:
                         if id(A) not in X:
                             X[id(A)] = torch.rand(A.cols, k, device="cuda") * 2 - 1
                         Y = torch.empty(A.rows, k, device="cuda")
-                        items.append(BatchItem(A, X[id(A)], Y, sched, ipu))
-                times[f"{sched}:{ipu}"] = _time_batch(items)
+                        items.append(BatchItem(A, X[id(A)], Y, sched, ipt))
+                times[f"{sched}:{ipt}"] = _time_batch(items)
             best = min(times, key=times.get)
             sched, tile = best.split(":")
             table.data["spmm"][key] = {"schedule": sched, "tile": tile, "us": times,

@@ -33,12 +33,12 @@ def _mats():
 MATS = _mats()
 # (matrix, k, schedule, items_per_thread)
 SPEC = [(0, 1, "merge_path_flat", 4), (1, 1, "merge_path_flat", 8), (2, 1, "merge_path_flat", 16),
-        (3, 1, "thread_mapped", 0), (4, 2, "merge_path_flat", 0), (5, 5, "merge_path_flat", 64),
-        (6, 1, "merge_path_flat", 8), (6, 16, "merge_path_flat", 256), (1, 33, "merge_path_flat", 128),
-        (2, 64, "merge_path_flat", 0), (0, 100, "merge_path_flat", 512), (7, 1, "merge_path_flat", 8),
+        (3, 1, "thread_mapped", 0), (4, 2, "merge_path_flat", 0), (5, 5, "merge_path_flat", 4),
+        (6, 1, "merge_path_flat", 8), (6, 16, "merge_path_flat", 16), (1, 33, "merge_path_flat", 8),
+        (2, 64, "merge_path_flat", 0), (0, 100, "merge_path_flat", 16), (7, 1, "merge_path_flat", 8),
         (7, 3, "merge_path_flat", 0), (6, 1, "thread_mapped", 0), (3, 1, "group_mapped", 0),
         (6, 1, "group_mapped", 64), (1, 1, "group_mapped", 2), (7, 1, "group_mapped", 0),
-        (4, 1, "group_mapped", 16), (6, 32, "merge_path_flat", 64)]
+        (4, 1, "group_mapped", 16), (6, 32, "merge_path_flat", 4), (5, 300, "merge_path_flat", 8)]
 
 
 def _items(vt="f32", seed=0):

@@ -33,13 +33,18 @@ def test_layout_counts():
     assert st == _capi.SUCCESS and info.work_items == 4 and info.spmv_tiles == 0
     st, info = query([item(1000, 9000, schedule="group_mapped", ipt=16)])
     assert st == _capi.SUCCESS and info.work_items == 63
-    st, info = query([item(1000, 9000, k=8)])  # 40 units of 256 steps, 8 lanes per unit
+    # k = 8: ranges of 8 lanes, 32 ranges x 8 steps = 256 steps per tile
+    st, info = query([item(1000, 9000, k=8)])
     assert st == _capi.SUCCESS
-    assert (info.spmm_units, info.spmm_work_items, info.work_items) == (40, 2, 2)
+    assert (info.spmm_units, info.spmm_work_items, info.work_items) == (40, 40, 40)
+    assert info.coordinates == 41
+    # k = 32 with 16-byte rows: 4 columns per lane, 8 lanes -> 256-step tiles at ipt 8
+    st, info = query([item(1000, 9000, k=32, ipt=16)])
+    assert st == _capi.SUCCESS and info.spmm_units == 20
     st, info = query([item(1000, 9000, ipt=4), item(1000, 9000, k=8), item(50, 0, ipt=16)])
     assert st == _capi.SUCCESS
-    assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 2
-    assert info.work_items == 13 and info.max_items_per_thread == 16
+    assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 41 + 2
+    assert info.work_items == 51 and info.max_items_per_thread == 16
     assert info.table_bytes > 0 and info.workspace_bytes > 0
 
 
@@ -49,7 +54,7 @@ def test_layout_rejections():
     assert query([item(1000, 9000, schedule="group_mapped", ipt=3)])[0] == NS
     assert query([item(1000, 9000, schedule="work_oriented")])[0] == NS
     assert query([item(1000, 9000, ipt=5)])[0] == NS
-    assert query([item(1000, 9000, k=4, ipt=8)])[0] == NS          # SpMM unit < 16 steps
+    assert query([item(1000, 9000, k=4, ipt=32)])[0] == NS         # SpMM tile: 4, 8, 16
     assert query([item(1000, 9000, k=0)])[0] == IV
     assert query([item(1000, 9000, k=8, ldx=4)])[0] == IV          # ldx < k
     assert query([item(-1, 0)])[0] == IV

@@ -81,12 +81,20 @@ def test_spmm_leading_dimension(schedule, k, pad):
     assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30)
 
 
-@pytest.mark.parametrize("ipu", [16, 64, 300, 1024, 4096])
-def test_spmm_merge_path_unit_sizes(ipu):
+@pytest.mark.parametrize("k", [1, 2, 3, 7, 16, 33, 64, 300])
+@pytest.mark.parametrize("ipt", [4, 8, 16])
+def test_spmm_merge_path_tile_sizes(ipt, k):
+    """Merge-path SpMM tiles of 4, 8 or 16 steps per range, lane groups of
+    2 .. 64 lanes, 1 or 4 columns per lane, column blocks when k > 256."""
     off, idx, val, cols = MATS["powerlaw_20k"]
-    C, Bh, _ = _run("merge_path", "powerlaw_20k", 64, items_per_thread=ipu)
+    C, Bh, _ = _run("merge_path", "powerlaw_20k", k, items_per_thread=ipt)
     C64, A64 = oracle.spmm_f64(off, idx, val, Bh)
     assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30)
+
+
+def test_spmm_merge_path_rejects_other_tiles():
+    with pytest.raises(loops.LoopsError):
+        _run("merge_path", "powerlaw_20k", 8, items_per_thread=256)
 
 
 def test_spmm_deterministic():

@@ -110,6 +110,7 @@ class BatchInfo(ctypes.Structure):
         ("spmm_work_items", ctypes.c_int64),
         ("coordinates", ctypes.c_int64),
         ("max_items_per_thread", ctypes.c_int64),
+        ("spmm_variant_mask", ctypes.c_int64),
     ]
 
 

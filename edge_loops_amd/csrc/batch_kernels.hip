@@ -1,9 +1,10 @@
 // Batches of independent products (many matrices, SpMV and multi-RHS SpMM
 // mixed) in one persistent kernel.
 //
-// The host fills a table (loops_batch_prepare): one descriptor per item
-// (pointers, sizes, the schedule chosen for it) and a work list of
-// workgroup-sized pieces:
+// The host fills a table (loops_batch_prepare) whose size grows with the
+// number of items, not with their sizes: one descriptor per item (pointers,
+// sizes, the schedule chosen for it) and, per kind of piece, the prefix
+// counts of each item's pieces.  The pieces are workgroup-sized:
 //   MERGE  one merge tile of an SpMV item (256 lanes x ipt steps, ipt 4/8/16),
 //          the same tile code as the single-matrix merge-path kernel;
 //   ROWS   256 consecutive rows of an SpMV item, one row per lane (the
@@ -14,6 +15,10 @@
 //   SPMM   one merge-path SpMM tile of an SpMM item (the single-matrix SpMM
 //          tile: ranges of G lanes spanning its columns, 1 or 4 columns per
 //          lane, IPT = 4, 8 or 16 steps per range).
+// The pieces of all items form one concatenated range per kernel; a piece
+// finds its item by a search over the prefix counts (a 64-lane cooperative
+// search, done again only when a workgroup's next piece leaves the item of
+// its previous one).
 // The caller copies the table to the device and calls loops_batch_launch:
 // a prologue kernel zeroes the work-queue counters and writes every merge
 // tile's start coordinate (one lane per coordinate), a persistent SpMV
@@ -29,6 +34,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "loops_internal.hpp"
@@ -38,6 +44,13 @@
 namespace loops_mi355x {
 
 enum batch_kind : int { BK_MERGE = 0, BK_ROWS = 1, BK_SPMM = 2, BK_GROUP = 3 };
+
+// SpMM tile variants, one persistent kernel each: value type x columns per
+// lane (1, 4) x merge steps per range (4, 8, 16).
+constexpr int kSpmmVariants = 3 * 2 * 3;
+__host__ __device__ constexpr int spmm_variant(int vtype, int cpl, int ipt) {
+  return (vtype * 2 + (cpl == 4 ? 1 : 0)) * 3 + (ipt == 4 ? 0 : ipt == 8 ? 1 : 2);
+}
 
 struct batch_item_dev {
   const int* offsets;
@@ -56,16 +69,63 @@ struct batch_item_dev {
   int ti, pad0, pad1, pad2;                 // ti: merge steps per tile (MERGE, SPMM)
 };
 
-struct batch_work {
-  int item, kind, sub, pad;
+// Prefix arrays of the table, n_items + 1 entries each: entry i = pieces /
+// slots / coordinates of items 0 .. i-1.
+enum batch_prefix : int {
+  BP_SPMV = 0,   // pieces of the SpMV kernel
+  BP_TILE = 1,   // merge-tile fix-up slots (MERGE items)
+  BP_COORD = 2,  // merge-tile start coordinates (MERGE and SPMM items)
+  BP_UNIT = 3,   // SpMM tile fix-up slots (SPMM items)
+  BP_COUNT = 4
 };
 
-// Work list: SpMV pieces in [0, n_spmv_work), SpMM pieces after them.
+// SpMM items are also listed grouped by tile variant (spmm_order, stable in
+// item order): variant v owns entries [var_begin[v], var_begin[v + 1]) of
+// spmm_order, and spmm_prefix[j] counts the pieces of entries before j.
 struct batch_header {
-  long long n_items, n_work, n_spmv_work, n_tiles, n_coords, n_units;
-  long long items_off, work_off, tile_item_off, coord_item_off, unit_item_off;
-  unsigned long long* queue;  // work-queue counters: SpMV kernel, then SpMM
+  long long n_items, n_spmv_work, n_spmm_work, n_tiles, n_coords, n_units;
+  long long items_off, prefix_off, order_off, sprefix_off, n_spmm_items;
+  long long var_begin[kSpmmVariants + 1];
+  unsigned long long* queue;  // work-queue counter sets: SpMV kernel, then each variant
 };
+
+__device__ __forceinline__ const long long* batch_prefix_of(const char* table,
+                                                            const batch_header* h, int kind) {
+  return reinterpret_cast<const long long*>(table + h->prefix_off) + kind * (h->n_items + 1);
+}
+
+// Item holding index i of a concatenated range: the largest j < n with
+// P[j] <= i (P non-decreasing, P[0] = 0 <= i < P[n]), so empty items are
+// skipped.  One lane's binary search.
+__device__ __forceinline__ int batch_find_item(const long long* __restrict__ P, long long n,
+                                               long long i) {
+  long long lo = 0, hi = n;  // P[lo] <= i; the answer is in [lo, hi)
+  while (hi - lo > 1) {
+    const long long mid = (lo + hi) >> 1;
+    if (P[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return (int)lo;
+}
+
+// The same, by a whole wavefront: each round 64 lanes probe 64 evenly spaced
+// entries of [lo, hi) and the count of probes <= i narrows the range to
+// one step (two rounds for up to 4096 items).  i must be wave-uniform.
+__device__ __forceinline__ int batch_wave_find_item(const long long* __restrict__ P,
+                                                    long long n, long long i) {
+  const int lane = threadIdx.x & 63;
+  long long lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const long long step = (hi - lo + 63) / 64;
+    const long long pos = lo + lane * step;
+    const bool le = pos < hi && P[pos] <= i;
+    const int c = __popcll(__ballot(le));  // >= 1: lane 0 probes lo
+    lo += (long long)(c - 1) * step;
+    const long long nh = lo + step;
+    hi = nh < hi ? nh : hi;
+  }
+  return (int)lo;
+}
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -81,7 +141,9 @@ constexpr int kBatchTpb = 256;
 #endif
 constexpr int kQueues = LOOPS_BATCH_QUEUES;
 constexpr int kQueueStride = 32;  // unsigned long longs between counters
-constexpr size_t kQueueBytes = 2 * kQueues * kQueueStride * sizeof(unsigned long long);
+constexpr int kQueueSets = 1 + kSpmmVariants;  // the SpMV kernel, then each variant
+constexpr size_t kQueueBytes =
+    (size_t)kQueueSets * kQueues * kQueueStride * sizeof(unsigned long long);
 
 // Next piece of workgroup blockIdx.x: subset q = blockIdx.x % nq holds
 // pieces q, q + nq, ...; its wq workgroups start with the first wq of them
@@ -143,74 +205,49 @@ __device__ __forceinline__ void run_group(const batch_item_dev& it, int block) {
   if (row < it.rows && lane == 0) it.y[row] = sum;
 }
 
-using batch_spmm_smem = spmm_smem<16, 4>;  // the largest SpMM tile's LDS
-
-template <int IPT, int CPL, typename V>
-__device__ __forceinline__ void run_spmm(const batch_item_dev& it, int tile,
-                                         batch_spmm_smem& raw) {
-  auto& sm = *reinterpret_cast<spmm_smem<IPT, CPL>*>(&raw);
-  spmm_tile<IPT, CPL, V>(it.offsets, it.indices, static_cast<const V*>(it.values), it.rows,
-                         it.nnz, it.k, it.x, it.ldx, it.y, it.ldy, it.g, tile, it.coords,
-                         it.sx, sm);
-}
-
 template <typename V, int MAXIPT>
-__device__ __forceinline__ void run_spmv_piece(const batch_item_dev& it,
-                                               const batch_work& w,
+__device__ __forceinline__ void run_spmv_piece(const batch_item_dev& it, int sub,
                                                mp_smem<kBatchTpb, MAXIPT>& sm) {
-  switch (w.kind) {
+  switch (it.kind) {
     case BK_MERGE:
       if constexpr (MAXIPT >= 16) {
-        if (it.ipt == 16) { run_merge<16, V, MAXIPT>(it, w.sub, sm); break; }
+        if (it.ipt == 16) { run_merge<16, V, MAXIPT>(it, sub, sm); break; }
       }
       if constexpr (MAXIPT >= 8) {
-        if (it.ipt == 8) { run_merge<8, V, MAXIPT>(it, w.sub, sm); break; }
+        if (it.ipt == 8) { run_merge<8, V, MAXIPT>(it, sub, sm); break; }
       }
-      run_merge<4, V, MAXIPT>(it, w.sub, sm);
+      run_merge<4, V, MAXIPT>(it, sub, sm);
       break;
     case BK_ROWS:
-      run_rows<V>(it, w.sub);
+      run_rows<V>(it, sub);
       break;
     case BK_GROUP:
       switch (it.g) {
-        case 2: run_group<2, V>(it, w.sub); break;
-        case 4: run_group<4, V>(it, w.sub); break;
-        case 8: run_group<8, V>(it, w.sub); break;
-        case 16: run_group<16, V>(it, w.sub); break;
-        case 32: run_group<32, V>(it, w.sub); break;
-        default: run_group<64, V>(it, w.sub); break;
+        case 2: run_group<2, V>(it, sub); break;
+        case 4: run_group<4, V>(it, sub); break;
+        case 8: run_group<8, V>(it, sub); break;
+        case 16: run_group<16, V>(it, sub); break;
+        case 32: run_group<32, V>(it, sub); break;
+        default: run_group<64, V>(it, sub); break;
       }
       break;
-  }
-}
-
-template <typename V>
-__device__ __forceinline__ void run_spmm_piece(const batch_item_dev& it, const batch_work& w,
-                                               batch_spmm_smem& sm) {
-  if (it.cpl == 4) {
-    if (it.ipt == 4) run_spmm<4, 4, V>(it, w.sub, sm);
-    else if (it.ipt == 16) run_spmm<16, 4, V>(it, w.sub, sm);
-    else run_spmm<8, 4, V>(it, w.sub, sm);
-  } else {
-    if (it.ipt == 4) run_spmm<4, 1, V>(it, w.sub, sm);
-    else if (it.ipt == 16) run_spmm<16, 1, V>(it, w.sub, sm);
-    else run_spmm<8, 1, V>(it, w.sub, sm);
   }
 }
 
 // Prologue: zero the work-queue counters; write the start coordinate of
 // every merge tile of every MERGE and SPMM item (and the end of its last
-// tile): coordinate c belongs to item coord_item[c].
+// tile).
 __global__ __launch_bounds__(256) void batch_prologue_kernel(const char* __restrict__ table) {
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const int* coord_item = reinterpret_cast<const int*>(table + h->coord_item_off);
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < 2 * kQueues) h->queue[c * kQueueStride] = 0;
+  if (c < kQueueSets * kQueues) h->queue[c * kQueueStride] = 0;
   if (c >= h->n_coords) return;
-  const batch_item_dev& it = items[coord_item[c]];
-  const long long t = c - it.first_coord;
+  const long long* P = batch_prefix_of(table, h, BP_COORD);
+  const int item = batch_find_item(P, h->n_items, c);
+  const batch_item_dev& it = items[item];
+  const long long t = c - P[item];
   const long long total = (long long)it.rows + it.nnz;
   long long d = t * (long long)it.ti;
   if (d > total) d = total;
@@ -221,7 +258,10 @@ __global__ __launch_bounds__(256) void batch_prologue_kernel(const char* __restr
 
 // Persistent kernels (see piece_queue).  Lane 0 claims the next piece as
 // the current one starts, so the atomic's latency overlaps the piece; every
-// workgroup leaves once its claim reaches n.
+// workgroup leaves once its claim reaches n.  Each wavefront keeps the
+// piece range [lo, hi) of its current item and searches the prefix counts
+// only when a piece falls outside it (the piece index is workgroup-uniform,
+// so every wavefront finds the same item).
 template <int MAXIPT>
 __global__ __launch_bounds__(kBatchTpb) void batch_spmv_kernel(const char* __restrict__ table) {
   __shared__ mp_smem<kBatchTpb, MAXIPT> sm;
@@ -229,18 +269,24 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmv_kernel(const char* __res
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const batch_work* work = reinterpret_cast<const batch_work*>(table + h->work_off);
+  const long long* P = batch_prefix_of(table, h, BP_SPMV);
   const long long n = h->n_spmv_work;
   const piece_queue pq(h->queue);
-  long long i = pq.first();
+  long long i = pq.first(), lo = 0, hi = 0;
+  int item = 0;
   while (i < n) {
     long long claim = 0;
     if (threadIdx.x == 0) claim = pq.claim();
-    const batch_work w = work[i];
-    const batch_item_dev& it = items[w.item];
-    if (it.vtype == LOOPS_VALUE_BF16) run_spmv_piece<bf16_t, MAXIPT>(it, w, sm);
-    else if (it.vtype == LOOPS_VALUE_F16) run_spmv_piece<f16_t, MAXIPT>(it, w, sm);
-    else run_spmv_piece<float, MAXIPT>(it, w, sm);
+    if (i < lo || i >= hi) {
+      item = batch_wave_find_item(P, h->n_items, i);
+      lo = P[item];
+      hi = P[item + 1];
+    }
+    const batch_item_dev& it = items[item];
+    const int sub = (int)(i - lo);
+    if (it.vtype == LOOPS_VALUE_BF16) run_spmv_piece<bf16_t, MAXIPT>(it, sub, sm);
+    else if (it.vtype == LOOPS_VALUE_F16) run_spmv_piece<f16_t, MAXIPT>(it, sub, sm);
+    else run_spmv_piece<float, MAXIPT>(it, sub, sm);
     if (threadIdx.x == 0) s_next = claim;
     __syncthreads();
     i = s_next;
@@ -248,24 +294,42 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmv_kernel(const char* __res
   }
 }
 
+// One instantiation per SpMM tile variant; it works through the pieces of
+// that variant's items only, so its registers and LDS are those of the
+// single-matrix kernel of the same tile.
+template <int IPT, int CPL, typename V>
 __global__ __launch_bounds__(kBatchTpb) void batch_spmm_kernel(const char* __restrict__ table) {
-  __shared__ batch_spmm_smem sm;
+  __shared__ spmm_smem<IPT, CPL> sm;
   __shared__ long long s_next;
+  constexpr int var = spmm_variant(
+      std::is_same<V, float>::value ? LOOPS_VALUE_F32
+                                    : std::is_same<V, bf16_t>::value ? LOOPS_VALUE_BF16
+                                                                     : LOOPS_VALUE_F16,
+      CPL, IPT);
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const batch_work* work = reinterpret_cast<const batch_work*>(table + h->work_off);
-  const long long n = h->n_work - h->n_spmv_work;
-  const piece_queue pq(h->queue + kQueues * kQueueStride);
-  long long i = pq.first();
+  const int* order = reinterpret_cast<const int*>(table + h->order_off);
+  const long long jb = h->var_begin[var], je = h->var_begin[var + 1];
+  const long long* P = reinterpret_cast<const long long*>(table + h->sprefix_off) + jb;
+  const long long base = P[0], n = P[je - jb] - base;
+  const piece_queue pq(h->queue + (1 + var) * kQueues * kQueueStride);
+  long long i = pq.first(), lo = 0, hi = 0;
+  int item = 0;
   while (i < n) {
     long long claim = 0;
     if (threadIdx.x == 0) claim = pq.claim();
-    const batch_work w = work[h->n_spmv_work + i];
-    const batch_item_dev& it = items[w.item];
-    if (it.vtype == LOOPS_VALUE_BF16) run_spmm_piece<bf16_t>(it, w, sm);
-    else if (it.vtype == LOOPS_VALUE_F16) run_spmm_piece<f16_t>(it, w, sm);
-    else run_spmm_piece<float>(it, w, sm);
+    const long long g = base + i;
+    if (g < lo || g >= hi) {
+      const int j = batch_wave_find_item(P, je - jb, g);
+      item = order[jb + j];
+      lo = P[j];
+      hi = P[j + 1];
+    }
+    const batch_item_dev& it = items[item];
+    spmm_tile<IPT, CPL, V>(it.offsets, it.indices, static_cast<const V*>(it.values), it.rows,
+                           it.nnz, it.k, it.x, it.ldx, it.y, it.ldy, it.g, (int)(g - lo),
+                           it.coords, it.sx, sm);
     if (threadIdx.x == 0) s_next = claim;
     __syncthreads();
     i = s_next;
@@ -273,20 +337,21 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmm_kernel(const char* __res
   }
 }
 
-// Merge-tile fix-up across the batch: slot s belongs to item tile_item[s].
+// Merge-tile fix-up across the batch: global slot s belongs to the item
+// found in the BP_TILE prefix counts.
 __global__ __launch_bounds__(256) void batch_spmv_fixup_kernel(const char* __restrict__ table) {
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const int* tile_item = reinterpret_cast<const int*>(table + h->tile_item_off);
+  const long long* P = batch_prefix_of(table, h, BP_TILE);
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   int item = 0;
   long long t = 0, t1 = -1;
   int r = -1;
   if (s < h->n_tiles) {
-    item = tile_item[s];
-    t = s - items[item].first_slot;
+    item = batch_find_item(P, h->n_items, s);
+    t = s - P[item];
     t1 = items[item].fx.seg_end[t];
     if (t1 >= 0) r = items[item].fx.carry_row[t];
   }
@@ -306,17 +371,30 @@ __global__ __launch_bounds__(256) void batch_spmv_fixup_kernel(const char* __res
   }
 }
 
-// SpMM tile fix-up across the batch: one workgroup per global tile slot.
-__global__ __launch_bounds__(64) void batch_spmm_fixup_kernel(const char* __restrict__ table) {
+// SpMM tile fix-up across the batch: one lane per global tile slot probes
+// its tile; its wavefront then finishes the rows found, one after another.
+__global__ __launch_bounds__(256) void batch_spmm_fixup_kernel(const char* __restrict__ table) {
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const int2* unit_item = reinterpret_cast<const int2*>(table + h->unit_item_off);
-  const long long s = blockIdx.x;
-  if (s >= h->n_units) return;
-  const int2 iu = unit_item[s];
-  const batch_item_dev& it = items[iu.x];
-  spmm_fixup_unit(it.offsets, it.rows, it.k, it.ti, iu.y, it.sx, it.y, it.ldy);
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int item = 0, r = -1;
+  long long t = 0, t_end = 0;
+  if (s < h->n_units) {
+    const long long* P = batch_prefix_of(table, h, BP_UNIT);
+    item = batch_find_item(P, h->n_items, s);
+    const batch_item_dev& it = items[item];
+    t = s - P[item];
+    r = spmm_fixup_probe(it.offsets, it.rows, it.ti, t, it.sx, &t_end);
+  }
+  unsigned long long todo = __ballot(r >= 0);
+  while (todo) {
+    const int src = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    const batch_item_dev& it = items[__shfl(item, src, 64)];
+    spmm_fixup_row(it.k, __shfl(t, src, 64), __shfl(t_end, src, 64), __shfl(r, src, 64),
+                   it.sx, it.y, it.ldy);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -359,86 +437,130 @@ inline spmm_pick spmm_shape_for(const loops_batch_item_t& b) {
 }
 }  // namespace
 
+namespace {
+// What one item contributes to a batch.
+struct item_counts {
+  int kind = BK_MERGE, ipt = 0, g = 0, cpl = 0, ti = 0, variant = -1;
+  long long spmv_work = 0, spmm_work = 0, tiles = 0, coords = 0, units = 0;
+  size_t spmm_ws = 0;
+};
+
+loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
+  const loops_csr_view_t& A = b.A;
+  if (A.rows < 0 || A.cols < 0 || A.nnzs < 0 || b.k < 1 || b.k > (1 << 20))
+    return LOOPS_ERROR_INVALID_VALUE;
+  if (A.rows + A.nnzs >= (int64_t)INT32_MAX || A.cols >= (int64_t)INT32_MAX)
+    return LOOPS_ERROR_OVERFLOW;
+  if (A.value_type != LOOPS_VALUE_F32 && A.value_type != LOOPS_VALUE_BF16 &&
+      A.value_type != LOOPS_VALUE_F16)
+    return LOOPS_ERROR_NOT_SUPPORTED;
+  if ((A.rows > 0 && (!A.offsets || !b.y)) || (A.nnzs > 0 && (!A.indices || !A.values || !b.x)))
+    return LOOPS_ERROR_INVALID_VALUE;
+  const uintptr_t va = A.value_type == LOOPS_VALUE_F32 ? 16 : 8;
+  if (reinterpret_cast<uintptr_t>(A.indices) % 16 || reinterpret_cast<uintptr_t>(A.values) % va)
+    return LOOPS_ERROR_NOT_SUPPORTED;  // batch pieces use 16-byte loads
+  if (b.k > 1 && (b.ldx < b.k || b.ldy < b.k)) return LOOPS_ERROR_INVALID_VALUE;
+  const long long total = A.rows + A.nnzs;
+  *c = item_counts{};
+  if (b.k == 1 && b.schedule == LOOPS_SCHEDULE_THREAD_MAPPED) {
+    c->kind = BK_ROWS;
+    c->spmv_work = (A.rows + kBatchTpb - 1) / kBatchTpb;
+  } else if (b.k == 1 && b.schedule == LOOPS_SCHEDULE_GROUP_MAPPED) {
+    c->kind = BK_GROUP;
+    c->g = group_lanes(b);
+    if (!valid_group(c->g)) return LOOPS_ERROR_NOT_SUPPORTED;
+    const long long per = kBatchTpb / c->g;
+    c->spmv_work = (A.rows + per - 1) / per;
+  } else if (b.k == 1) {
+    if (b.schedule != LOOPS_SCHEDULE_MERGE_PATH_FLAT) return LOOPS_ERROR_NOT_SUPPORTED;
+    c->kind = BK_MERGE;
+    c->ipt = b.items_per_thread ? b.items_per_thread : 8;
+    if (c->ipt != 4 && c->ipt != 8 && c->ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
+    c->ti = kBatchTpb * c->ipt;
+    const long long T = (total + c->ti - 1) / c->ti;
+    c->spmv_work = T;
+    c->tiles = T;
+    c->coords = T + 1;
+  } else {
+    if (b.schedule != LOOPS_SCHEDULE_MERGE_PATH_FLAT) return LOOPS_ERROR_NOT_SUPPORTED;
+    c->kind = BK_SPMM;
+    c->ipt = b.items_per_thread ? b.items_per_thread : 8;
+    if (c->ipt != 4 && c->ipt != 8 && c->ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
+    const spmm_pick sp = spmm_shape_for(b);
+    c->g = sp.g;
+    c->cpl = sp.cpl;
+    c->ti = (int)spmm_tile_items(c->g, c->ipt);
+    const long long U = (total + c->ti - 1) / c->ti;
+    c->spmm_work = U;
+    c->units = U;
+    c->coords = U + 1;
+    c->spmm_ws = U > 0 ? a256(spmm_ws_bytes_impl(U, (int)b.k)) : 0;
+    c->variant = spmm_variant(A.value_type, c->cpl, c->ipt);
+  }
+  return LOOPS_SUCCESS;
+}
+}  // namespace
+
 // Sizes of the table and of the device workspace for these items; with
-// out_table, also fills the table (host memory) pointing into dev_ws.
+// out_table, also fills the table (host memory) pointing into dev_ws.  The
+// table holds the header, one descriptor per item and BP_COUNT prefix
+// arrays of n + 1 counts: O(n) bytes whatever the sizes of the items.
 loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
                             loops_batch_info_t* info, batch_item_dev* out_table,
                             char* dev_ws) {
   long long n_spmv_work = 0, n_spmm_work = 0, n_tiles = 0, n_coords = 0, n_units = 0;
   int max_ipt = 0;
   size_t spmm_ws_bytes = 0;
-  // First pass: validation, counts and workspace sizes.
+  long long var_count[kSpmmVariants] = {};
+  long long n_spmm_items = 0;
+  int64_t var_mask = 0;
+  item_counts c;
   for (int64_t i = 0; i < n; ++i) {
-    const loops_batch_item_t& b = items[i];
-    const loops_csr_view_t& A = b.A;
-    if (A.rows < 0 || A.cols < 0 || A.nnzs < 0 || b.k < 1 || b.k > (1 << 20))
-      return LOOPS_ERROR_INVALID_VALUE;
-    if (A.rows + A.nnzs >= (int64_t)INT32_MAX || A.cols >= (int64_t)INT32_MAX)
-      return LOOPS_ERROR_OVERFLOW;
-    if (A.value_type != LOOPS_VALUE_F32 && A.value_type != LOOPS_VALUE_BF16 &&
-        A.value_type != LOOPS_VALUE_F16)
-      return LOOPS_ERROR_NOT_SUPPORTED;
-    if ((A.rows > 0 && (!A.offsets || !b.y)) ||
-        (A.nnzs > 0 && (!A.indices || !A.values || !b.x)))
-      return LOOPS_ERROR_INVALID_VALUE;
-    const uintptr_t va = A.value_type == LOOPS_VALUE_F32 ? 16 : 8;
-    if (reinterpret_cast<uintptr_t>(A.indices) % 16 ||
-        reinterpret_cast<uintptr_t>(A.values) % va)
-      return LOOPS_ERROR_NOT_SUPPORTED;  // batch pieces use 16-byte loads
-    if (b.k > 1 && (b.ldx < b.k || b.ldy < b.k)) return LOOPS_ERROR_INVALID_VALUE;
-    const long long total = A.rows + A.nnzs;
-    if (b.k == 1 && b.schedule == LOOPS_SCHEDULE_THREAD_MAPPED) {
-      n_spmv_work += (A.rows + kBatchTpb - 1) / kBatchTpb;
-    } else if (b.k == 1 && b.schedule == LOOPS_SCHEDULE_GROUP_MAPPED) {
-      const int g = group_lanes(b);
-      if (!valid_group(g)) return LOOPS_ERROR_NOT_SUPPORTED;
-      n_spmv_work += (A.rows + kBatchTpb / g - 1) / (kBatchTpb / g);
-    } else if (b.k == 1) {
-      if (b.schedule != LOOPS_SCHEDULE_MERGE_PATH_FLAT) return LOOPS_ERROR_NOT_SUPPORTED;
-      const int ipt = b.items_per_thread ? b.items_per_thread : 8;
-      if (ipt != 4 && ipt != 8 && ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
-      const long long T = (total + (long long)kBatchTpb * ipt - 1) / ((long long)kBatchTpb * ipt);
-      n_spmv_work += T;
-      n_tiles += T;
-      n_coords += T + 1;
-      if (ipt > max_ipt) max_ipt = ipt;
-    } else {
-      if (b.schedule != LOOPS_SCHEDULE_MERGE_PATH_FLAT) return LOOPS_ERROR_NOT_SUPPORTED;
-      const int ipt = b.items_per_thread ? b.items_per_thread : 8;
-      if (ipt != 4 && ipt != 8 && ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
-      const long long ti = spmm_tile_items(spmm_shape_for(b).g, ipt);
-      const long long U = (total + ti - 1) / ti;
-      n_spmm_work += U;
-      n_units += U;
-      n_coords += U + 1;
-      spmm_ws_bytes += U > 0 ? a256(spmm_ws_bytes_impl(U, (int)b.k)) : 0;
+    const loops_status_t st = count_item(items[i], &c);
+    if (st != LOOPS_SUCCESS) return st;
+    if (c.kind == BK_SPMM) {
+      ++var_count[c.variant];
+      ++n_spmm_items;
+      if (c.spmm_work > 0) var_mask |= (int64_t)1 << c.variant;
     }
+    n_spmv_work += c.spmv_work;
+    n_spmm_work += c.spmm_work;
+    n_tiles += c.tiles;
+    n_coords += c.coords;
+    n_units += c.units;
+    spmm_ws_bytes += c.spmm_ws;
+    if (c.kind == BK_MERGE && c.ipt > max_ipt) max_ipt = c.ipt;
   }
-  if (n_spmv_work + n_spmm_work >= (long long)INT32_MAX || n_tiles >= INT32_MAX ||
-      n_coords >= INT32_MAX)
+  if (n_spmv_work >= (long long)INT32_MAX || n_spmm_work >= (long long)INT32_MAX ||
+      n_tiles >= INT32_MAX || n_coords >= INT32_MAX || n_units >= INT32_MAX ||
+      n >= INT32_MAX)
     return LOOPS_ERROR_OVERFLOW;
   const size_t tile_ws = n_tiles > 0 ? a256(fixup_ws_bytes(n_tiles)) : 0;
   const size_t coord_ws = a256((size_t)n_coords * sizeof(loops_coord_t));
   batch_header hd{};
   hd.n_items = n;
-  hd.n_work = n_spmv_work + n_spmm_work;
   hd.n_spmv_work = n_spmv_work;
+  hd.n_spmm_work = n_spmm_work;
   hd.n_tiles = n_tiles;
   hd.n_coords = n_coords;
   hd.n_units = n_units;
+  hd.n_spmm_items = n_spmm_items;
+  hd.var_begin[0] = 0;
+  for (int v = 0; v < kSpmmVariants; ++v) hd.var_begin[v + 1] = hd.var_begin[v] + var_count[v];
   hd.items_off = a256(sizeof(batch_header));
-  hd.work_off = hd.items_off + a256((size_t)n * sizeof(batch_item_dev));
-  hd.tile_item_off = hd.work_off + a256((size_t)hd.n_work * sizeof(batch_work));
-  hd.coord_item_off = hd.tile_item_off + a256((size_t)n_tiles * sizeof(int));
-  hd.unit_item_off = hd.coord_item_off + a256((size_t)n_coords * sizeof(int));
-  info->table_bytes = (int64_t)(hd.unit_item_off + a256((size_t)n_units * 2 * sizeof(int)));
+  hd.prefix_off = hd.items_off + a256((size_t)n * sizeof(batch_item_dev));
+  hd.order_off = hd.prefix_off + a256((size_t)BP_COUNT * (n + 1) * sizeof(long long));
+  hd.sprefix_off = hd.order_off + a256((size_t)n_spmm_items * sizeof(int));
+  info->table_bytes =
+      (int64_t)(hd.sprefix_off + a256((size_t)(n_spmm_items + 1) * sizeof(long long)));
   info->workspace_bytes = (int64_t)(kQueueBytes + tile_ws + coord_ws + spmm_ws_bytes + 256);
-  info->work_items = hd.n_work;
+  info->work_items = n_spmv_work + n_spmm_work;
   info->spmv_tiles = n_tiles;
   info->spmm_units = n_units;
   info->spmm_work_items = n_spmm_work;
   info->coordinates = n_coords;
   info->max_items_per_thread = max_ipt;
+  info->spmm_variant_mask = var_mask;
   if (!out_table) return LOOPS_SUCCESS;
 
   // Second pass: fill the table.
@@ -447,16 +569,24 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
   dev_ws += kQueueBytes;
   std::memcpy(table, &hd, sizeof(hd));
   batch_item_dev* its = reinterpret_cast<batch_item_dev*>(table + hd.items_off);
-  batch_work* work = reinterpret_cast<batch_work*>(table + hd.work_off);
-  int* tile_item = reinterpret_cast<int*>(table + hd.tile_item_off);
-  int* coord_item = reinterpret_cast<int*>(table + hd.coord_item_off);
-  int* unit_item = reinterpret_cast<int*>(table + hd.unit_item_off);
+  long long* pre = reinterpret_cast<long long*>(table + hd.prefix_off);
+  long long* P[BP_COUNT];
+  for (int q = 0; q < BP_COUNT; ++q) P[q] = pre + q * (n + 1);
+  int* order = reinterpret_cast<int*>(table + hd.order_off);
+  long long* SP = reinterpret_cast<long long*>(table + hd.sprefix_off);
+  long long var_next[kSpmmVariants];
+  for (int v = 0; v < kSpmmVariants; ++v) var_next[v] = hd.var_begin[v];
   fixup_ws all_tiles = n_tiles > 0 ? carve_fixup_ws(dev_ws, n_tiles) : fixup_ws{};
   loops_coord_t* all_coords = reinterpret_cast<loops_coord_t*>(dev_ws + tile_ws);
   char* spmm_base = dev_ws + tile_ws + coord_ws;
-  long long wv = 0, wm = n_spmv_work, slot = 0, coord = 0, unit = 0;
+  long long wv = 0, wm = 0, slot = 0, coord = 0, unit = 0;
   for (int64_t i = 0; i < n; ++i) {
     const loops_batch_item_t& b = items[i];
+    (void)count_item(b, &c);  // validated in the first pass
+    P[BP_SPMV][i] = wv;
+    P[BP_TILE][i] = slot;
+    P[BP_COORD][i] = coord;
+    P[BP_UNIT][i] = unit;
     batch_item_dev d{};
     d.offsets = b.A.offsets;
     d.indices = b.A.indices;
@@ -470,70 +600,79 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
     d.nnz = (int)b.A.nnzs;
     d.k = (int)b.k;
     d.vtype = b.A.value_type;
-    const long long total = b.A.rows + b.A.nnzs;
-    if (b.k == 1 && b.schedule == LOOPS_SCHEDULE_THREAD_MAPPED) {
-      d.kind = BK_ROWS;
-      const long long nb = (b.A.rows + kBatchTpb - 1) / kBatchTpb;
-      for (long long q = 0; q < nb; ++q) work[wv++] = batch_work{(int)i, BK_ROWS, (int)q, 0};
-    } else if (b.k == 1 && b.schedule == LOOPS_SCHEDULE_GROUP_MAPPED) {
-      d.kind = BK_GROUP;
-      d.g = group_lanes(b);
-      const long long per = kBatchTpb / d.g;
-      const long long nb = (b.A.rows + per - 1) / per;
-      for (long long q = 0; q < nb; ++q) work[wv++] = batch_work{(int)i, BK_GROUP, (int)q, 0};
-    } else if (b.k == 1) {
-      d.kind = BK_MERGE;
-      d.ipt = b.items_per_thread ? b.items_per_thread : 8;
-      d.ti = kBatchTpb * d.ipt;
-      const long long T = (total + d.ti - 1) / d.ti;
-      d.first_slot = (int)slot;
+    d.kind = c.kind;
+    d.ipt = c.ipt;
+    d.g = c.g;
+    d.cpl = c.cpl;
+    d.ti = c.ti;
+    d.first_slot = (int)slot;
+    d.first_coord = (int)coord;
+    if (c.kind == BK_MERGE) {
       d.fx.carry_row = all_tiles.carry_row + slot;
       d.fx.seg_end = all_tiles.seg_end + slot;
       d.fx.carry_val = all_tiles.carry_val + slot;
       d.fx.head_val = all_tiles.head_val + slot;
-      d.first_coord = (int)coord;
-      d.coords = all_coords + coord;
-      for (long long t = 0; t < T; ++t) {
-        work[wv++] = batch_work{(int)i, BK_MERGE, (int)t, 0};
-        tile_item[slot + t] = (int)i;
-      }
-      for (long long t = 0; t <= T; ++t) coord_item[coord + t] = (int)i;
-      slot += T;
-      coord += T + 1;
-    } else {
-      d.kind = BK_SPMM;
-      d.ipt = b.items_per_thread ? b.items_per_thread : 8;
-      const spmm_pick sp = spmm_shape_for(b);
-      d.g = sp.g;
-      d.cpl = sp.cpl;
-      d.ti = (int)spmm_tile_items(d.g, d.ipt);
-      const long long U = (total + d.ti - 1) / d.ti;
-      d.units = (int)U;
-      if (U > 0) {
-        d.sx = carve_spmm_ws(spmm_base, U, (int)b.k);
-        spmm_base += a256(spmm_ws_bytes_impl(U, (int)b.k));
-      }
-      d.first_coord = (int)coord;
-      d.coords = all_coords + coord;
-      for (long long t = 0; t <= U; ++t) coord_item[coord + t] = (int)i;
-      coord += U + 1;
-      for (long long q = 0; q < U; ++q) work[wm++] = batch_work{(int)i, BK_SPMM, (int)q, 0};
-      for (long long q = 0; q < U; ++q) {
-        unit_item[2 * (unit + q)] = (int)i;
-        unit_item[2 * (unit + q) + 1] = (int)q;
-      }
-      unit += U;
     }
+    if (c.coords > 0) d.coords = all_coords + coord;
+    if (c.kind == BK_SPMM) {
+      order[var_next[c.variant]++] = (int)i;
+      d.units = (int)c.units;
+      if (c.units > 0) {
+        d.sx = carve_spmm_ws(spmm_base, c.units, (int)b.k);
+        spmm_base += c.spmm_ws;
+      }
+    }
+    wv += c.spmv_work;
+    wm += c.spmm_work;
+    slot += c.tiles;
+    coord += c.coords;
+    unit += c.units;
     its[i] = d;
   }
+  P[BP_SPMV][n] = wv;
+  // SpMM pieces in variant order: each variant's pieces are contiguous.
+  long long sp = 0;
+  for (long long j = 0; j < n_spmm_items; ++j) {
+    SP[j] = sp;
+    sp += its[order[j]].units;
+  }
+  SP[n_spmm_items] = sp;
+  (void)wm;
+  P[BP_TILE][n] = slot;
+  P[BP_COORD][n] = coord;
+  P[BP_UNIT][n] = unit;
   return LOOPS_SUCCESS;
 }
 
 namespace {
+using spmm_kernel_fn = void (*)(const char*);
+
+// The SpMM kernel of each variant, indexed by spmm_variant().
+template <typename V>
+void spmm_kernels_of(spmm_kernel_fn* k) {
+  k[0] = batch_spmm_kernel<4, 1, V>;
+  k[1] = batch_spmm_kernel<8, 1, V>;
+  k[2] = batch_spmm_kernel<16, 1, V>;
+  k[3] = batch_spmm_kernel<4, 4, V>;
+  k[4] = batch_spmm_kernel<8, 4, V>;
+  k[5] = batch_spmm_kernel<16, 4, V>;
+}
+
+struct spmm_kernel_table {
+  spmm_kernel_fn fn[kSpmmVariants];
+  spmm_kernel_table() {
+    spmm_kernels_of<float>(fn + 6 * LOOPS_VALUE_F32);
+    spmm_kernels_of<bf16_t>(fn + 6 * LOOPS_VALUE_BF16);
+    spmm_kernels_of<f16_t>(fn + 6 * LOOPS_VALUE_F16);
+  }
+};
+const spmm_kernel_table g_spmm_kernels;
+
 // Resident workgroups of each persistent kernel on a device (occupancy x CUs),
 // computed once per device.
 struct batch_grids {
-  int spmv4 = 0, spmv8 = 0, spmv16 = 0, spmm = 0;
+  int spmv4 = 0, spmv8 = 0, spmv16 = 0;
+  int spmm[kSpmmVariants] = {};
 };
 constexpr int kMaxDevices = 64;
 batch_grids g_grids[kMaxDevices];
@@ -542,11 +681,11 @@ hipError_t resident_grids(batch_grids* out) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  if (dev >= 0 && dev < kMaxDevices && g_grids[dev].spmm > 0) {
+  if (dev >= 0 && dev < kMaxDevices && g_grids[dev].spmv4 > 0) {
     *out = g_grids[dev];
     return hipSuccess;
   }
-  int cus = 0, a4 = 0, a8 = 0, a16 = 0, b = 0;
+  int cus = 0, a4 = 0, a8 = 0, a16 = 0;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a4, batch_spmv_kernel<4>, kBatchTpb, 0);
@@ -554,14 +693,18 @@ hipError_t resident_grids(batch_grids* out) {
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a8, batch_spmv_kernel<8>, kBatchTpb, 0);
   if (e == hipSuccess)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a16, batch_spmv_kernel<16>, kBatchTpb, 0);
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, batch_spmm_kernel, kBatchTpb, 0);
   if (e != hipSuccess) return e;
   batch_grids g;
+  for (int v = 0; v < kSpmmVariants; ++v) {
+    int b = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &b, reinterpret_cast<const void*>(g_spmm_kernels.fn[v]), kBatchTpb, 0);
+    if (e != hipSuccess) return e;
+    g.spmm[v] = (b > 0 ? b : 1) * cus;
+  }
   g.spmv4 = (a4 > 0 ? a4 : 1) * cus;
   g.spmv8 = (a8 > 0 ? a8 : 1) * cus;
   g.spmv16 = (a16 > 0 ? a16 : 1) * cus;
-  g.spmm = (b > 0 ? b : 1) * cus;
   if (dev >= 0 && dev < kMaxDevices) g_grids[dev] = g;
   *out = g;
   return hipSuccess;
@@ -582,7 +725,7 @@ hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
   const long long spmv_work = info->work_items - info->spmm_work_items;
   if (info->work_items > 0) {
     const long long pg = info->coordinates > 0 ? (info->coordinates + 255) / 256 : 1;
-    static_assert(2 * kQueues <= 256, "prologue's first block zeroes the counters");
+    static_assert(kQueueSets * kQueues <= 256, "prologue's first block zeroes the counters");
     batch_prologue_kernel<<<(unsigned)pg, 256, 0, s>>>(t);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -597,10 +740,11 @@ hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (info->spmm_work_items > 0) {
-    long long g = grid > 0 ? grid : rg.spmm;
+  for (int v = 0; v < kSpmmVariants; ++v) {
+    if (!((info->spmm_variant_mask >> v) & 1)) continue;
+    long long g = grid > 0 ? grid : rg.spmm[v];
     if (g > info->spmm_work_items) g = info->spmm_work_items;
-    batch_spmm_kernel<<<(unsigned)g, kBatchTpb, 0, s>>>(t);
+    g_spmm_kernels.fn[v]<<<(unsigned)g, kBatchTpb, 0, s>>>(t);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -610,7 +754,7 @@ hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
     if (e != hipSuccess) return e;
   }
   if (info->spmm_units > 0) {
-    batch_spmm_fixup_kernel<<<(unsigned)info->spmm_units, 64, 0, s>>>(t);
+    batch_spmm_fixup_kernel<<<(unsigned)((info->spmm_units + 255) / 256), 256, 0, s>>>(t);
     return hipGetLastError();
   }
   return hipSuccess;

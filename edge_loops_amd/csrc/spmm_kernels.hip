@@ -109,13 +109,22 @@ __global__ __launch_bounds__(TPB) void spmm_group_mapped_kernel(
   }
 }
 
-__global__ void spmm_fixup_kernel(const int* __restrict__ offsets, int rows,
-                                  int k, long long ti, long long tiles,
-                                  spmm_ws ws, float* __restrict__ C,
-                                  long long ldc) {
-  const long long t = blockIdx.x;
-  if (t >= tiles) return;
-  spmm_fixup_unit(offsets, rows, k, ti, t, ws, C, ldc);
+// One wavefront per 64 consecutive tiles: every lane probes its tile, then
+// the wavefront finishes the rows found, one after another.
+__global__ __launch_bounds__(256) void spmm_fixup_kernel(const int* __restrict__ offsets,
+                                                         int rows, int k, long long ti,
+                                                         long long tiles, spmm_ws ws,
+                                                         float* __restrict__ C, long long ldc) {
+  const long long t0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) & ~63LL;
+  const long long t = t0 + (threadIdx.x & 63);
+  long long t_end = 0;
+  const int r = t < tiles ? spmm_fixup_probe(offsets, rows, ti, t, ws, &t_end) : -1;
+  unsigned long long todo = __ballot(r >= 0);
+  while (todo) {
+    const int src = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    spmm_fixup_row(k, t0 + src, __shfl(t_end, src, 64), __shfl(r, src, 64), ws, C, ldc);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -188,7 +197,7 @@ hipError_t mp_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
   e = record(ev1, s);
   if (e != hipSuccess) return e;
   if (tiles > 1) {
-    spmm_fixup_kernel<<<dim3((unsigned)tiles), dim3(64), 0, s>>>(
+    spmm_fixup_kernel<<<dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, s>>>(
         A.offsets, A.rows, k, ti, tiles, w, C, ldc);
     e = hipGetLastError();
   }

@@ -330,28 +330,33 @@ __global__ __launch_bounds__(kSpmmTpb) void spmm_merge_path_kernel(
                          blockIdx.x, nullptr, ws, sm);
 }
 
-// Per (tile, column): the tile where an open row starts sums the tile
-// partials of that row in tile order (see spmv_fixup_kernel); ti = merge
-// steps per tile.  Called by a whole workgroup for tile t; its lanes stride
-// over the columns.
-__device__ __forceinline__ void spmm_fixup_unit(const int* __restrict__ offsets,
-                                                int rows, int k, long long ti,
-                                                long long t, const spmm_ws& ws,
-                                                float* __restrict__ C,
-                                                long long ldc) {
+// Fix-up of rows that cross tiles.  A row r open across tiles t .. t_end
+// (t = the tile where it starts, fr / ti with fr = offsets[r] + r its first
+// merge step) is finished as carry[t] + carry[t+1] + ... + head[t_end] per
+// column, in tile order.
+//
+// spmm_fixup_probe: for tile t, the row to finish there (or -1) and t_end.
+__device__ __forceinline__ int spmm_fixup_probe(const int* __restrict__ offsets, int rows,
+                                                long long ti, long long t,
+                                                const spmm_ws& ws, long long* t_end) {
   const int r = ws.carry_row[t];
-  if (r < 0 || r >= rows) return;
+  if (r < 0 || r >= rows) return -1;
   const long long fr = (long long)offsets[r] + r;
-  if (fr / ti != t) return;
-  const long long t_end = ((long long)offsets[r + 1] + r + 1 - 1) / ti;
-  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+  if (fr / ti != t) return -1;
+  *t_end = ((long long)offsets[r + 1] + r) / ti;
+  return r;
+}
+
+// spmm_fixup_row: the calling wavefront's lanes stride over the k columns.
+__device__ __forceinline__ void spmm_fixup_row(int k, long long t, long long t_end, int r,
+                                               const spmm_ws& ws, float* __restrict__ C,
+                                               long long ldc) {
+  for (int c = threadIdx.x & 63; c < k; c += 64) {
     float s = ws.carry[t * k + c];
     for (long long v = t + 1; v < t_end; ++v) s += ws.carry[v * k + c];
     s += ws.head[t_end * k + c];
     C[(long long)r * ldc + c] = s;
   }
 }
-
-
 
 }  // namespace loops_mi355x

@@ -241,6 +241,8 @@ typedef struct loops_batch_info_t {
   int64_t spmm_work_items; /* pieces of the SpMM items (of work_items)  */
   int64_t coordinates;     /* merge-tile coordinates written per launch */
   int64_t max_items_per_thread; /* largest merge tile of the SpMV items   */
+  int64_t spmm_variant_mask; /* bit v: SpMM pieces of tile variant v exist;
+                                one persistent kernel is launched per bit  */
 } loops_batch_info_t;
 
 loops_status_t loops_batch_query(const loops_batch_item_t* items, int64_t n,

@@ -7,6 +7,11 @@ to 200,000; columns uniform and sorted per row; int32 indices; fp32 values
 and x uniform [-1, 1).  Synthetic data (seeded generator), resident in HBM
 before timing.
 
+Unless --quick / --no-forms, the line also carries the batched form (512
+matrices in one persistent launch vs per-matrix plain launches) and the
+served form (continuous batching over 64 resident matrices vs plain
+per-call launches: req/s, p50/p99), see edge_loops_amd/bench_forms.py.
+
 A step is one execution of the matrix's merge-path SpMV plan
 (loops_spmv_plan_execute: the tiles of 4 column panels of A in one launch,
 the row fix-up, the panel combine), built once per matrix beforehand (its
@@ -47,10 +52,14 @@ def parse():
     ap.add_argument("--tpb", type=int, default=0)
     ap.add_argument("--ipt", type=int, default=0)
     ap.add_argument("--panels", type=int, default=0, help="column panels (0: automatic)")
+    ap.add_argument("--layout", default="auto", choices=["auto", "panels", "blocks"],
+                    help="SpMV plan layout (auto: panels)")
     ap.add_argument("--schedules", default="all",
                     help="comma list of extra schedules to time, 'all' or 'none'")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="skip extras (profiling runs)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched (512 matrices) and served (64 matrices) legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (--steps times); "
                          "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -118,12 +127,16 @@ def main():
         tuning["items_per_thread"] = args.ipt
     torch.cuda.synchronize()
     t0 = time.time()
-    plans = [loops.spmv_plan(A, panels=args.panels, stream=stream, **tuning) for A in copies]
+    plans = [loops.spmv_plan(A, panels=args.panels, stream=stream, layout=args.layout, **tuning)
+             for A in copies]
     torch.cuda.synchronize()
     build_ms = (time.time() - t0) * 1e3 / len(plans)
-    log(f"plans: {plans[0].panels} panels, {plans[0].device_bytes/1e6:.0f} MB each, "
+    log(f"plans: {plans[0].layout}, {plans[0].panels} panels/segments, "
+        f"{plans[0].device_bytes/1e6:.0f} MB each, "
         f"{build_ms:.1f} ms to build")
 
+    n_panels = plans[0].panels
+    layout = plans[0].layout
     nbytes = synth.spmv_bytes(R, C, N, 4)
     flops = 2 * N
 
@@ -149,7 +162,8 @@ def main():
     log(f"cold: {ms_per_step*1e3:.1f} us/step, kernel {kern_avg*1e3:.1f} us, "
         f"{value:.1f} GFLOP/s, {nbytes/kern_avg/1e6:.0f} GB/s")
 
-    extras = {"plan": {"panels": plans[0].panels, "build_ms": round(build_ms, 2),
+    extras = {"plan": {"layout": layout, "panels": plans[0].panels,
+                       "build_ms": round(build_ms, 2),
                        "device_bytes": plans[0].device_bytes}}
     if not args.quick:
         # warm: the same copy every step
@@ -226,6 +240,15 @@ def main():
     cpu_mt_s = statistics.median(runs_mt)
     log(f"cpu: 1 core {cpu_s*1e3:.1f} ms, {mt} threads {cpu_mt_s*1e3:.1f} ms")
 
+    if not args.quick and not args.no_forms:
+        # The batched and served forms against plain per-call launches.
+        from edge_loops_amd import bench_forms
+        del copies, plans
+        torch.cuda.empty_cache()
+        extras["batched"] = bench_forms.batched(log=log)
+        torch.cuda.empty_cache()
+        extras["served"] = bench_forms.served(log=log)
+
     achieved = nbytes / (kern_avg * 1e-3) / 1e9
     out = {
         "metric": "SpMV GFLOP/s + % of HBM roofline; served req/s & p50/p99 vs plain per-call",
@@ -244,8 +267,8 @@ def main():
             "workload": "c2_powerlaw_2m merge_path_flat SpMV",
             "rows": R, "cols": C, "nnz": N, "max_row": A_h.max_row(),
             "zipf": 1.8, "index": "int32", "values": "f32",
-            "threads_per_block": args.tpb or 512, "items_per_thread": args.ipt or 4,
-            "column_panels": plans[0].panels,
+            "plan_layout": layout,
+            "column_panels" if layout == "panels" else "column_segments": n_panels,
             "rotated_copies": len(copies), "parallelism": "single-gpu",
         },
         "roofline": {
@@ -255,7 +278,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profile(),
-            "kernel": "spmv_panels_kernel" if plans[0].panels > 1 else "spmv_merge_path_kernel",
+            "kernel": ("spmv_blocks_kernel" if layout == "blocks" else
+                       "spmv_panels_kernel" if n_panels > 1 else "spmv_merge_path_kernel"),
             "kernel_avg_ms": round(kern_avg, 5),
             "algorithmic_bytes": nbytes,
         },

@@ -83,7 +83,7 @@ class PlanParams(ctypes.Structure):
         ("threads_per_block", ctypes.c_int32),
         ("items_per_thread", ctypes.c_int32),
         ("panels", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("layout", ctypes.c_int32),
     ]
 
 
@@ -113,6 +113,9 @@ class BatchInfo(ctypes.Structure):
         ("spmm_variant_mask", ctypes.c_int64),
     ]
 
+
+# loops_plan_layout_t
+PLAN_LAYOUTS = {"auto": 0, "panels": 1, "blocks": 2}
 
 # loops_server_mode_t
 SERVER_BATCHED = 0
@@ -188,6 +191,7 @@ SIGNATURES = {
                                      _vp, ctypes.POINTER(ctypes.c_void_p)]),
     "loops_spmv_plan_execute": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "loops_spmv_plan_info": (_st, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
+    "loops_spmv_plan_layout": (_st, [_vp, ctypes.POINTER(_i32)]),
     "loops_spmv_plan_destroy": (_st, [_vp]),
     "loops_batch_query": (_st, [ctypes.POINTER(BatchItem), _i64, ctypes.POINTER(BatchInfo)]),
     "loops_batch_prepare": (_st, [ctypes.POINTER(BatchItem), _i64, _vp, ctypes.c_size_t, _vp,

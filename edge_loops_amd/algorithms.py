@@ -164,17 +164,21 @@ class spmm:
 
 
 class spmv_plan:
-    """A merge-path SpMV plan for one matrix (loops_spmv_plan_create): tile
-    coordinates computed once and, with panels > 1, A copied into column
-    panels that run on separate XCDs.  ``execute(x, y)`` is asynchronous and
-    allocates nothing."""
+    """An SpMV plan for one matrix (loops_spmv_plan_create).  layout
+    "panels": merge-path tile coordinates computed once and, with panels >
+    1, A copied into column panels that run on separate XCDs
+    (deterministic); "blocks": A copied into row blocks x column segments
+    with y accumulated in LDS (LDS atomics: last bits vary between runs);
+    "auto": panels.  ``execute(x, y)`` is
+    asynchronous and allocates nothing."""
 
     def __init__(self, csr, panels=0, threads_per_block=0, items_per_thread=0,
-                 stream=None):
+                 stream=None, layout="auto"):
         self.csr = csr  # keeps A alive: with one panel the plan reads it in place
         self.rows, self.cols = csr.rows, csr.cols
         v = csr.view()
-        p = _capi.PlanParams(int(threads_per_block), int(items_per_thread), int(panels), 0)
+        p = _capi.PlanParams(int(threads_per_block), int(items_per_thread), int(panels),
+                             _capi.PLAN_LAYOUTS[layout])
         h = ctypes.c_void_p()
         _capi.check(_capi.lib().loops_spmv_plan_create(
             _capi.handle(), ctypes.byref(v), ctypes.byref(p), _stream(stream),
@@ -184,6 +188,10 @@ class spmv_plan:
         _capi.check(_capi.lib().loops_spmv_plan_info(h, ctypes.byref(n), ctypes.byref(b)),
                     "loops_spmv_plan_info")
         self.panels, self.device_bytes = int(n.value), int(b.value)
+        lay = ctypes.c_int32()
+        _capi.check(_capi.lib().loops_spmv_plan_layout(h, ctypes.byref(lay)),
+                    "loops_spmv_plan_layout")
+        self.layout = {v: k for k, v in _capi.PLAN_LAYOUTS.items()}[int(lay.value)]
 
     def execute(self, x, y, stream=None, start_event=None, stop_event=None):
         _capi.check(_capi.lib().loops_spmv_plan_execute(

@@ -10,6 +10,8 @@ whole passes of all 512 products, inputs resident in HBM:
   batched_merge8  the same with merge_path_flat 256 x 8 for every matrix;
   plain_eager     512 loops_spmv calls (merge_path_flat, caller workspace,
                   precomputed nothing), launched back to back on one stream;
+  plain_blocking  the same calls, each waited for before the next (the
+                  reference's host entries synchronise before returning);
   plain_graph     the same 512 calls captured once in a graph and replayed.
 """
 from __future__ import annotations
@@ -102,6 +104,15 @@ def batched(count=512, reps=10, table=None, log=print):
 
     ms = _time(plain, max(2, reps // 3))
     out["plain_eager"] = leg(ms)
+
+    def plain_blocking():
+        for A, x, y, w in zip(dev, xs, ys, wss):
+            run_spmv("merge_path_flat", A, x, y, workspace=w, threads_per_block=256,
+                     items_per_thread=8)
+            torch.cuda.current_stream().synchronize()
+
+    out["plain_blocking"] = leg(_time(plain_blocking, 2))
+    log(f"plain_blocking: {out['plain_blocking']['ms_per_pass']:.3f} ms/pass")
     log(f"plain_eager: {ms:.3f} ms/pass")
     same = all(torch.equal(a, b) for a, b in zip(results["batched_merge8"], ys))
     out["batched_merge8_bitwise_equal_plain"] = bool(same)
@@ -115,7 +126,7 @@ def batched(count=512, reps=10, table=None, log=print):
     out["plain_graph"] = leg(ms)
     log(f"plain_graph: {ms:.3f} ms/pass")
     del g
-    for k in ("plain_eager", "plain_graph"):
+    for k in ("plain_eager", "plain_graph", "plain_blocking"):
         out[f"speedup_tuned_vs_{k}"] = out[k]["ms_per_pass"] / out["batched_tuned"]["ms_per_pass"]
     return out
 
@@ -157,9 +168,9 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
            "pool_rows": int(sum(m.rows for m in mats)), "requests": n_requests,
            "mean_k": float(ks.mean()), "slots": slots}
 
-    def server(mode, vt):
+    def server(mode, vt, plain_slots=64):
         srv = Server(mode=mode, arena_bytes=32 << 30, scratch_bytes=1 << 30,
-                     slots=slots if mode == "batched" else 64)
+                     slots=slots if mode == "batched" else plain_slots)
         dev, tun = pools[vt]
         ids = [srv.add_matrix(A, tunings=t) for A, t in zip(dev, tun)]
         reqs = [(ids[m], x1[m] if k == 1 else X32[m][:, :k]) for m, k in zip(mid, ks)]
@@ -176,6 +187,11 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
         return r
 
     res = {}
+    # Blocking per-call: one request on the GPU at a time, as with the
+    # reference's host entry points, which synchronise before returning.
+    srv, reqs = server("plain", "f32", plain_slots=1)
+    res["plain_blocking_f32_burst"] = run(srv, reqs, None)
+    srv.close()
     srv, reqs = server("plain", "f32")
     res["plain_f32_burst"] = run(srv, reqs, None)
     cap_plain = res["plain_f32_burst"]["requests_per_s"]
@@ -199,5 +215,7 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
             f"{res[f'batched_{vt}_at_3x_load']['p99_ms']:.2f} ms")
     out.update(res)
     out["capacity_ratio_f32"] = res["batched_f32_burst"]["requests_per_s"] / cap_plain
+    out["capacity_ratio_f32_vs_blocking"] = (res["batched_f32_burst"]["requests_per_s"] /
+                                             res["plain_blocking_f32_burst"]["requests_per_s"])
     out["offered_load_per_s"] = lam0
     return out

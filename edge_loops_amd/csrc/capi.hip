@@ -301,9 +301,30 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
   if (st != LOOPS_SUCCESS) return st;
   const loops_plan_params_t zero{};
   const loops_plan_params_t* p = params ? params : &zero;
-  if (p->panels < 0 || p->panels > kMaxPanels || p->reserved != 0 ||
-      p->threads_per_block < 0 || p->items_per_thread < 0)
+  if (p->panels < 0 || p->panels > kMaxPanels || p->layout < LOOPS_PLAN_LAYOUT_AUTO ||
+      p->layout > LOOPS_PLAN_LAYOUT_BLOCKS || p->threads_per_block < 0 ||
+      p->items_per_thread < 0)
     return LOOPS_ERROR_INVALID_VALUE;
+  int layout = p->layout;
+  if (layout == LOOPS_PLAN_LAYOUT_AUTO) layout = LOOPS_PLAN_LAYOUT_PANELS;
+  if (layout == LOOPS_PLAN_LAYOUT_BLOCKS) {
+    loops_spmv_plan* pl = new (std::nothrow) loops_spmv_plan();
+    if (!pl) return LOOPS_ERROR_ALLOCATION;
+    pl->st.layout = LOOPS_PLAN_LAYOUT_BLOCKS;
+    pl->st.rows = (int)A->rows;
+    pl->rows = A->rows;
+    pl->cols = A->cols;
+    const hipError_t e = blk_plan_build(a, &pl->st.blk, stream);
+    if (e != hipSuccess) {
+      blk_plan_release(&pl->st.blk);
+      delete pl;
+      return e == hipErrorOutOfMemory ? LOOPS_ERROR_ALLOCATION : hip_status(e);
+    }
+    pl->st.P = pl->st.blk.S;
+    pl->st.device_bytes = pl->st.blk.device_bytes;
+    *plan = pl;
+    return LOOPS_SUCCESS;
+  }
   const int tpb = p->threads_per_block ? p->threads_per_block : 512;
   const int ipt = p->items_per_thread ? p->items_per_thread : 4;
   int panels = p->panels;
@@ -335,6 +356,13 @@ loops_status_t loops_spmv_plan_execute(loops_spmv_plan_t plan, const float* x,
                                        hipStream_t stream) {
   if (!plan) return LOOPS_ERROR_INVALID_VALUE;
   if ((plan->rows > 0 && !y) || (plan->cols > 0 && !x)) return LOOPS_ERROR_INVALID_VALUE;
+  if (plan->st.layout == LOOPS_PLAN_LAYOUT_BLOCKS) {
+    if (start_event) {
+      const hipError_t e = hipEventRecord(start_event, stream);
+      if (e != hipSuccess) return hip_status(e);
+    }
+    return hip_status(blk_plan_execute(plan->st.blk, x, y, stop_event, stream));
+  }
   return hip_status(plan_execute(&plan->st, x, y, start_event, stop_event, stream));
 }
 
@@ -346,8 +374,15 @@ loops_status_t loops_spmv_plan_info(loops_spmv_plan_t plan, int32_t* panels,
   return LOOPS_SUCCESS;
 }
 
+loops_status_t loops_spmv_plan_layout(loops_spmv_plan_t plan, int32_t* layout) {
+  if (!plan || !layout) return LOOPS_ERROR_INVALID_VALUE;
+  *layout = plan->st.layout;
+  return LOOPS_SUCCESS;
+}
+
 loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan) {
   if (!plan) return LOOPS_ERROR_INVALID_VALUE;
+  blk_plan_release(&plan->st.blk);
   plan_release(&plan->st);
   delete plan;
   return LOOPS_SUCCESS;

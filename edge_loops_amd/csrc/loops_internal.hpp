@@ -92,7 +92,31 @@ struct panel_set_t {
   int nnz[kMaxPanels];
   int tiles[kMaxPanels];
 };
+// ---- row-block plan (spmv_blocks.hip) ------------------------------------
+constexpr int kBlkRowBits = 14;
+constexpr int kBlkRows = 1 << kBlkRowBits;     // rows per block: y in LDS
+constexpr int kBlkColBits = 32 - kBlkRowBits;  // column bits within a segment
+constexpr int kBlkChunkCols = 64;              // columns per build bucket
+struct blk_plan_state {
+  int rows = 0, cols = 0, nnz = 0, S = 8, W = 1, nb = 0;
+  loops_value_type_t vtype = LOOPS_VALUE_F32;
+  uint32_t* packed = nullptr;  // column in segment << kBlkRowBits | row in block
+  void* values = nullptr;
+  int* stream_off = nullptr;   // nb * S + 1 entries
+  float* partial = nullptr;    // S arrays of part_stride floats
+  long long part_stride = 0;
+  void* mem = nullptr;
+  size_t device_bytes = 0;
+};
+int blk_segments(long long cols);
+hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s);
+hipError_t blk_plan_execute(const blk_plan_state& st, const float* x, float* y,
+                            hipEvent_t ev1, hipStream_t s);
+void blk_plan_release(blk_plan_state* st);
+
 struct plan_state {
+  int layout = LOOPS_PLAN_LAYOUT_PANELS;
+  blk_plan_state blk;
   int P = 1, rows = 0, tpb = 256, ipt = 8, tmax = 0;
   bool aligned = true;
   loops_value_type_t vtype = LOOPS_VALUE_F32;
@@ -109,6 +133,9 @@ hipError_t plan_build(const csr_args& A, int P, int tpb, int ipt, plan_state* st
 hipError_t plan_execute(plan_state* st, const float* x, float* y, hipEvent_t ev0,
                         hipEvent_t ev1, hipStream_t s);
 void plan_release(plan_state* st);
+size_t exclusive_scan_scratch_bytes(long long n);
+hipError_t launch_exclusive_scan(const int* cnt, long long n, int* out, void* scratch,
+                                 hipStream_t s);
 
 // ---- batches -------------------------------------------------------------
 struct batch_item_dev;

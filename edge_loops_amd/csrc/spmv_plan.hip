@@ -143,6 +143,25 @@ __global__ __launch_bounds__(256) void scan_write_kernel(
   if (b == 0 && threadIdx.x == 0) out[rows] = total[p];
 }
 
+// Exclusive scan of n ints (n >= 1): out[i] = cnt[0] + ... + cnt[i-1],
+// out[n] = total.  scratch: ((n + 2047) / 2048 + 1) ints + one int.
+size_t exclusive_scan_scratch_bytes(long long n) {
+  return ((size_t)((n + kScanItems - 1) / kScanItems) + 2) * sizeof(int);
+}
+
+hipError_t launch_exclusive_scan(const int* cnt, long long n, int* out, void* scratch,
+                                 hipStream_t s) {
+  const int nb = (int)((n + kScanItems - 1) / kScanItems);
+  int* bsum = static_cast<int*>(scratch);
+  int* total = bsum + nb + 1;
+  panel_ptrs_t po{};
+  po.p[0] = out;
+  scan_block_sums_kernel<<<dim3(nb, 1), 256, 0, s>>>(cnt, (int)n, nb, bsum);
+  scan_block_prefix_kernel<<<1, 1024, 0, s>>>(bsum, nb, total);
+  scan_write_kernel<<<dim3(nb, 1), 256, 0, s>>>(cnt, (int)n, nb, bsum, total, po);
+  return hipGetLastError();
+}
+
 // Copies every nonzero to its panel: row by binary search over offsets,
 // panel by column, position from the panel offsets and the row's counts.
 template <typename V>

@@ -169,21 +169,32 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
                                              hipStream_t stream);
 
 /* ---- SpMV plan: preprocessed once per matrix, executed many times ------
- * merge_path_flat SpMV with the tile coordinates computed once (the
- * reference's preprocess_t, merge_path_flat.hxx:150-230) and, with panels >
- * 1, A copied into `panels` column panels whose tiles run on separate XCDs
- * so each XCD's L2 holds only its slice of x (no counterpart in the
- * reference).  Creation allocates and synchronises `stream`; execution
- * allocates nothing and can be captured into a hipGraph.  With panels == 1
- * the plan reads A in place, so A must outlive it unchanged; with panels > 1
- * A is no longer needed after creation. */
+ * Layout PANELS: merge_path_flat SpMV with the tile coordinates computed
+ * once (the reference's preprocess_t, merge_path_flat.hxx:150-230) and,
+ * with panels > 1, A copied into `panels` column panels whose tiles run on
+ * separate XCDs so each XCD's L2 holds only its slice of x.  Deterministic.
+ * Layout BLOCKS: A copied into blocks of 16384 rows x column segments (one
+ * per XCD), each block's entries bucketed by 64-column chunks; a workgroup
+ * accumulates its block's y in LDS, so x is read in column order.  LDS
+ * float atomics: results vary in the last bits from run to run.
+ * Neither copied layout has a counterpart in the reference.  Creation
+ * allocates and synchronises `stream`; execution allocates nothing and can
+ * be captured into a hipGraph.  With PANELS and panels == 1 the plan reads
+ * A in place, so A must outlive it unchanged; otherwise A is no longer
+ * needed after creation. */
 typedef struct loops_spmv_plan* loops_spmv_plan_t;
 
+typedef enum loops_plan_layout_t {
+  LOOPS_PLAN_LAYOUT_AUTO = 0,   /* PANELS                                    */
+  LOOPS_PLAN_LAYOUT_PANELS = 1,
+  LOOPS_PLAN_LAYOUT_BLOCKS = 2
+} loops_plan_layout_t;
+
 typedef struct loops_plan_params_t {
-  int32_t threads_per_block; /* 0: 512                                      */
-  int32_t items_per_thread;  /* 0: 4                                        */
-  int32_t panels;            /* 1..8; 0: one per 2 MiB of x, at most 8      */
-  int32_t reserved;          /* must be 0                                   */
+  int32_t threads_per_block; /* PANELS: 0 = 512                             */
+  int32_t items_per_thread;  /* PANELS: 0 = 4                               */
+  int32_t panels;            /* PANELS: 1..8; 0 = one per 2 MiB of x, <= 8  */
+  int32_t layout;            /* loops_plan_layout_t                         */
 } loops_plan_params_t;
 
 loops_status_t loops_spmv_plan_create(loops_handle_t handle,
@@ -196,8 +207,11 @@ loops_status_t loops_spmv_plan_execute(loops_spmv_plan_t plan, const float* x,
                                        float* y, hipEvent_t start_event,
                                        hipEvent_t stop_event,
                                        hipStream_t stream);
+/* panels: PANELS: the column panels; BLOCKS: the column segments.
+ * layout (may be null): the layout chosen. */
 loops_status_t loops_spmv_plan_info(loops_spmv_plan_t plan, int32_t* panels,
                                     int64_t* device_bytes);
+loops_status_t loops_spmv_plan_layout(loops_spmv_plan_t plan, int32_t* layout);
 loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
 
 /* ---- Batches: many independent products in one persistent launch -------

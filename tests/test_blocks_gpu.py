@@ -1,0 +1,136 @@
+"""GPU parity of the row-block SpMV plan (layout "blocks"): A copied into
+blocks of 16384 rows x column segments (a multiple of 8, one per XCD), each
+(block, segment) stream bucketed by 64-column chunks; y accumulated in LDS
+with float atomics, the segments' partial rows added in segment order.
+
+Bar: per row |y - y64| <= 1e-5 * sum_j |a_ij x_j| against the FP64 oracle and
+the reference validator's count of |y - y32| > 1e-2 equal to 0
+(examples/spmv/helpers.hxx:97-146).  The summation order inside a block
+depends on scheduling, so repeated runs are compared with the same bound,
+not bitwise."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import random_csr
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import edge_loops_amd as loops  # noqa: E402
+from edge_loops_amd import synth  # noqa: E402
+
+
+def _mats():
+    m = {}
+    off, idx, val = random_csr(3000, 2500, 40, seed=1, long_rows=((7, 2500), (2999, 1700)))
+    m["skewed"] = (off, idx, val, 2500)
+    lens = np.array([3, 9001, 1, 7003, 2, 0, 4099, 5], np.int64)
+    off = np.zeros(lens.shape[0] + 1, np.int32)
+    np.cumsum(lens, out=off[1:])
+    rng = np.random.default_rng(4)
+    idx = np.concatenate([np.sort(rng.choice(12000, L, replace=False)) for L in lens]).astype(np.int32)
+    m["long_misaligned"] = (off, idx, rng.uniform(-1, 1, idx.shape[0]).astype(np.float32), 12000)
+    off, idx, val = random_csr(700, 5000, 3, seed=2, empty_frac=0.7)
+    m["mostly_empty"] = (off, idx, val, 5000)
+    m["no_nnz"] = (np.zeros(129, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), 7)
+    off, idx, val = random_csr(40, 5, 5, seed=3, empty_frac=0.1)
+    m["five_cols"] = (off, idx, val, 5)
+    B = synth.powerlaw_csr(60000, 80000, 1.8, 1, 20000, 600000, seed=9, force_first_max=True)
+    m["powerlaw_60k"] = (B.offsets, B.indices, B.values, B.cols)
+    # several row blocks, the last one partial; 16 column segments
+    B = synth.powerlaw_csr(50000, 3_000_000, 2.0, 1, 30000, 500000, seed=11,
+                           force_first_max=True)
+    m["wide_3m_cols"] = (B.offsets, B.indices, B.values, B.cols)
+    return m
+
+
+MATS = _mats()
+
+
+def _check(y, off, idx, val, x):
+    y32 = oracle.spmv_f32(off, idx, val, x)
+    y64, a = oracle.spmv_f64(off, idx, val, x)
+    assert oracle.count_abs_errors(y, y32) == 0
+    bad, worst = oracle.count_rel_errors(y, y64, a, 1e-5)
+    assert bad == 0, worst
+
+
+@pytest.mark.parametrize("name", list(MATS))
+def test_blocks_parity(name):
+    off, idx, val, cols = MATS[name]
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+    plan = loops.spmv_plan(A, layout="blocks")
+    assert plan.layout == "blocks"
+    assert plan.panels == (16 if cols > 8 * 2 ** 18 else 8)
+    x_np = np.random.default_rng(3).uniform(-1, 1, cols).astype(np.float32)
+    x = torch.from_numpy(x_np).cuda()
+    for _ in range(2):
+        y = torch.full((A.rows,), float("nan"), device="cuda")
+        plan.execute(x, y)
+        torch.cuda.synchronize()
+        _check(y.cpu().numpy(), off, idx, val, x_np)
+    plan.destroy()
+
+
+def test_blocks_outlive_source():
+    off, idx, val, cols = MATS["powerlaw_60k"]
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+    plan = loops.spmv_plan(A, layout="blocks")
+    plan.csr = None
+    A.indices.fill_(0)
+    A.values.fill_(0)
+    del A
+    x_np = np.random.default_rng(5).uniform(-1, 1, cols).astype(np.float32)
+    y = torch.empty(off.shape[0] - 1, device="cuda")
+    plan.execute(torch.from_numpy(x_np).cuda(), y)
+    torch.cuda.synchronize()
+    _check(y.cpu().numpy(), off, idx, val, x_np)
+
+
+@pytest.mark.parametrize("vt", ["bf16", "f16"])
+def test_blocks_low_precision(vt):
+    off, idx, val, cols = MATS["powerlaw_60k"]
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val, value_type=vt)
+    stored = A.values.float().cpu().numpy()
+    x_np = np.random.default_rng(6).uniform(-1, 1, cols).astype(np.float32)
+    plan = loops.spmv_plan(A, layout="blocks")
+    y = torch.empty(A.rows, device="cuda")
+    plan.execute(torch.from_numpy(x_np).cuda(), y)
+    torch.cuda.synchronize()
+    y64, a = oracle.spmv_f64(off, idx, stored, x_np)
+    bad, worst = oracle.count_rel_errors(y.cpu().numpy(), y64, a, 1e-5)
+    assert bad == 0, worst
+
+
+def test_blocks_graph_replays():
+    off, idx, val, cols = MATS["powerlaw_60k"]
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+    plan = loops.spmv_plan(A, layout="blocks")
+    x_np = np.random.default_rng(8).uniform(-1, 1, cols).astype(np.float32)
+    x = torch.from_numpy(x_np).cuda()
+    y = torch.empty(A.rows, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        plan.execute(x, y, stream=s.cuda_stream)
+    for _ in range(3):
+        y.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        _check(y.cpu().numpy(), off, idx, val, x_np)
+
+
+def test_blocks_c2_full_size():
+    A_h = synth.c2_powerlaw_2m()
+    x_np = synth.uniform(A_h.cols, seed=21)
+    A = loops.csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values)
+    plan = loops.spmv_plan(A, layout="blocks")
+    assert plan.layout == "blocks" and plan.panels == 8
+    y = torch.empty(A_h.rows, device="cuda")
+    plan.execute(torch.from_numpy(x_np).cuda(), y)
+    torch.cuda.synchronize()
+    _check(y.cpu().numpy(), A_h.offsets, A_h.indices, A_h.values, x_np)

@@ -102,8 +102,8 @@ def test_plan_c2_full_size_auto_panels():
     A_h = synth.c2_powerlaw_2m()
     x_np = synth.uniform(A_h.cols, seed=21)
     A = loops.csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values)
-    plan = loops.spmv_plan(A)
-    assert plan.panels == 4  # 8 MB of x, one panel per 2 MiB
+    plan = loops.spmv_plan(A, layout="panels")
+    assert plan.panels == 4 and plan.layout == "panels"  # 8 MB of x, one panel per 2 MiB
     y = torch.empty(A_h.rows, device="cuda")
     plan.execute(torch.from_numpy(x_np).cuda(), y)
     torch.cuda.synchronize()

@@ -93,17 +93,24 @@ struct panel_set_t {
   int tiles[kMaxPanels];
 };
 // ---- row-block plan (spmv_blocks.hip) ------------------------------------
-constexpr int kBlkRowBits = 14;
-constexpr int kBlkRows = 1 << kBlkRowBits;     // rows per block: y in LDS
-constexpr int kBlkColBits = 32 - kBlkRowBits;  // column bits within a segment
-constexpr int kBlkChunkCols = 64;              // columns per build bucket
+// Two workgroups per CU: LDS y of kBlkRows floats + a round's products
+// (kBlkRound floats) each, within the CU's 160 KB.
+constexpr int kBlkRows = 15872;                 // rows per block (< 2^14)
+constexpr int kBlkSlotBits = 12;
+constexpr int kBlkRound = 1 << kBlkSlotBits;    // entries per round at most
+constexpr int kBlkColBits = 32 - kBlkSlotBits;  // column bits within a segment
+constexpr int kBlkChunkCols = 64;               // columns per build bucket
 struct blk_plan_state {
-  int rows = 0, cols = 0, nnz = 0, S = 8, W = 1, nb = 0;
+  int rows = 0, cols = 0, nnz = 0, S = 4, W = 1, nb = 0, nrounds = 0;
   loops_value_type_t vtype = LOOPS_VALUE_F32;
-  uint32_t* packed = nullptr;  // column in segment << kBlkRowBits | row in block
-  void* values = nullptr;
-  int* stream_off = nullptr;   // nb * S + 1 entries
-  float* partial = nullptr;    // S arrays of part_stride floats
+  uint32_t* packed = nullptr;   // column in segment << kBlkSlotBits | slot in round
+  void* values = nullptr;       // column order within the round
+  uint16_t* srow = nullptr;     // row in block of slot k of round r at r * kBlkRound + k
+  int* stream_round = nullptr;  // first round of each (block, segment) stream
+  int* round_off = nullptr;     // first entry of each round (nrounds + 1)
+  int* blk_start = nullptr;     // first row of each block (nb + 1)
+  int* blk_order = nullptr;     // blocks in launch order (heaviest first)
+  float* partial = nullptr;     // S arrays of part_stride floats
   long long part_stride = 0;
   void* mem = nullptr;
   size_t device_bytes = 0;

@@ -6,31 +6,45 @@
 // traffic bounds the merge-path kernel even when x sits in L2.  This plan
 // reorders A once so that the x reads of a wavefront share lines:
 //
-//   rows    are cut into blocks of kBlkRows rows; a workgroup keeps the y
-//           of its block in LDS (kBlkRows floats) and adds products into it
-//           with LDS atomics;
-//   columns are cut into S segments of W columns (S a multiple of 8), and
-//           workgroup b * S + s takes segment s of block b, so segment s
-//           runs on XCD s % 8 (workgroups go to the 8 XCDs round-robin) and
-//           each XCD's L2 holds only its slice of x;
+//   rows    are cut into blocks of at most kBlkRows rows and about equal
+//           nonzero counts (enough blocks for two waves of resident
+//           workgroups), launched heaviest first; a workgroup keeps the y
+//           of its block in LDS (kBlkRows floats), two workgroups per CU;
+//   columns are cut into S segments of W columns (S = 4 when 4 segments of
+//           2^kBlkColBits columns hold them, else a multiple of 8), and
+//           workgroup b * S + s takes segment s of block b, so a segment
+//           always runs on the same XCDs (workgroups go to the 8 XCDs
+//           round-robin) and each XCD's L2 holds only its slice of x;
 //   entries of each (block, segment) stream are bucketed by chunks of
-//           kBlkChunkCols columns, so a wavefront's 64 consecutive entries
-//           fall on a few x lines instead of 64.
+//           kBlkChunkCols columns and the stream is cut into rounds of
+//           whole chunks, at most kBlkRound entries each, so a wavefront's
+//           64 consecutive entries fall on a few x lines.
 //
-// Each entry is stored as its value and one 32-bit word: column within the
-// segment (kBlkColBits bits) above row within the block (kBlkRowBits bits)
-// -- 8 bytes per fp32 entry, as in CSR.  Every workgroup writes its block's
-// partial y for its segment; a combine kernel adds the S partials of each
-// row in segment order.  LDS float atomics add in an order that depends on
-// scheduling, so results are not bitwise reproducible from run to run (each
-// row's error stays within (n + 1) u sum|a x|, like any summation order);
-// the merge-path paths are the deterministic ones.
+// A round is processed in two phases.  Phase 1 reads its entries in column
+// order, multiplies by x and writes each product to its slot in LDS: the
+// plan stores, per entry, the column within the segment and the entry's
+// position ("slot") in the round sorted by (row, column).  Phase 2 lets lane
+// t sum slots 8t .. 8t + 7 (rows from the plan's per-slot row array), a
+// workgroup segmented scan carries row sums across lanes, and the lane that
+// holds a row's last slot adds the row's total to the LDS accumulator --
+// one writer per row per round, so no atomics, and every sum has an order
+// fixed by the plan: results are bitwise reproducible.  Every workgroup
+// writes its block's partial y for its segment; a combine kernel adds the S
+// partials of each row in segment order.
 //
-// Build (loops_spmv_plan_create, layout LOOPS_PLAN_LAYOUT_BLOCKS): count the
-// entries of every (block, segment, chunk) bucket, an exclusive scan over
-// the buckets, then a scatter of every entry into its bucket.
+// Bytes per fp32 entry: 4 (value) + 4 (column | slot) + 2 (row of a slot).
+// The loads of round r + 1 are issued while round r's x reads are in
+// flight, so each round waits for memory once.
+//
+// Build (loops_spmv_plan_create, layout LOOPS_PLAN_LAYOUT_BLOCKS): count
+// the entries of every (block, segment, chunk) bucket, an exclusive scan,
+// the rounds of every stream, a scatter of every entry into its bucket and
+// a sort of every round by (row, column).  Within a chunk the scatter's
+// order depends on scheduling; it matters only when one chunk of one block
+// holds more than kBlkRound entries (it is then split across rounds).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -44,7 +58,19 @@ namespace loops_mi355x {
 // ---------------------------------------------------------------------------
 struct blk_geom {
   int rows, cols, nnz, S, W, nch, nb;  // nch: chunks per segment; nb: row blocks
+  const int* blk_start;                // first row of each block (nb + 1)
 };
+
+// Block holding row r: the largest b with blk_start[b] <= r.
+__device__ __forceinline__ int blk_block_of(const blk_geom& g, int r) {
+  int lo = 0, hi = g.nb;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (g.blk_start[mid] <= r) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 __device__ __forceinline__ int blk_row_of(const int* __restrict__ offsets, int rows,
                                           long long nz) {
@@ -57,8 +83,7 @@ __device__ __forceinline__ int blk_row_of(const int* __restrict__ offsets, int r
   return lo;
 }
 
-__device__ __forceinline__ long long blk_bucket(const blk_geom& g, int r, int col) {
-  const int b = r >> kBlkRowBits;
+__device__ __forceinline__ long long blk_bucket(const blk_geom& g, int b, int col) {
   const int s = col / g.W;
   const int ch = (col - s * g.W) / kBlkChunkCols;
   return ((long long)b * g.S + s) * g.nch + ch;
@@ -71,113 +96,256 @@ __global__ __launch_bounds__(256) void blk_count_kernel(const int* __restrict__ 
   for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
        nz += (long long)gridDim.x * blockDim.x) {
     const int r = blk_row_of(offsets, g.rows, nz);
-    atomicAdd(&cnt[blk_bucket(g, r, indices[nz])], 1);
+    atomicAdd(&cnt[blk_bucket(g, blk_block_of(g, r), indices[nz])], 1);
   }
 }
 
-// Every nonzero to the next free slot of its bucket (cursor = bucket starts).
+// Rounds of stream q: its chunks in order, packed greedily into rounds of at
+// most kBlkRound entries; a chunk larger than that is cut into full rounds
+// and a remainder that later chunks may join.  With off == nullptr only
+// counts them (nrounds[q]); otherwise writes each round's first entry at
+// off[first[q] ...].
+__device__ __forceinline__ int blk_rounds_of(const int* __restrict__ start, long long q,
+                                             int nch, int* __restrict__ off) {
+  const long long k0 = q * nch;
+  int n = 0, cur = 0, cur_start = start[k0];
+  for (int c = 0; c < nch; ++c) {
+    const int a = start[k0 + c], size = start[k0 + c + 1] - a;
+    if (size == 0) continue;
+    if (cur + size <= kBlkRound) {
+      if (cur == 0) cur_start = a;
+      cur += size;
+      continue;
+    }
+    if (cur > 0) {
+      if (off) off[n] = cur_start;
+      ++n;
+    }
+    int done = 0;
+    while (size - done > kBlkRound) {
+      if (off) off[n] = a + done;
+      ++n;
+      done += kBlkRound;
+    }
+    cur = size - done;
+    cur_start = a + done;
+  }
+  if (cur > 0) {
+    if (off) off[n] = cur_start;
+    ++n;
+  }
+  return n;
+}
+
+__global__ void blk_round_count_kernel(const int* __restrict__ start, int nstreams, int nch,
+                                       int* __restrict__ nrounds) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nstreams) nrounds[q] = blk_rounds_of(start, q, nch, nullptr);
+}
+
+__global__ void blk_round_write_kernel(const int* __restrict__ start, int nstreams, int nch,
+                                       const int* __restrict__ first, int nnz,
+                                       int* __restrict__ round_off) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nstreams) blk_rounds_of(start, q, nch, round_off + first[q]);
+  if (q == nstreams) round_off[first[nstreams]] = nnz;
+}
+
+// Every nonzero to the next free slot of its bucket (cursor = bucket
+// starts): column within the segment to packed, row within the block to
+// trow (both in stream order until blk_round_sort_kernel).
 template <typename V>
 __global__ __launch_bounds__(256) void blk_scatter_kernel(
     const int* __restrict__ offsets, const int* __restrict__ indices,
     const V* __restrict__ values, blk_geom g, int* __restrict__ cursor,
-    uint32_t* __restrict__ packed, V* __restrict__ vals) {
+    uint32_t* __restrict__ packed, uint16_t* __restrict__ trow, V* __restrict__ vals) {
   for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
        nz += (long long)gridDim.x * blockDim.x) {
     const int r = blk_row_of(offsets, g.rows, nz);
+    const int b = blk_block_of(g, r);
     const int col = indices[nz];
-    const int pos = atomicAdd(&cursor[blk_bucket(g, r, col)], 1);
-    const int s = col / g.W;
-    packed[pos] = ((uint32_t)(col - s * g.W) << kBlkRowBits) |
-                  (uint32_t)(r & (kBlkRows - 1));
+    const int pos = atomicAdd(&cursor[blk_bucket(g, b, col)], 1);
+    packed[pos] = (uint32_t)(col - (col / g.W) * g.W);
+    trow[pos] = (uint16_t)(r - g.blk_start[b]);
     vals[pos] = values[nz];
   }
 }
 
-// stream_off[q] = start of the first bucket of stream q = (block, segment).
-__global__ void blk_streams_kernel(const int* __restrict__ starts, int nstreams, int nch,
-                                   int* __restrict__ stream_off) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q <= nstreams) stream_off[q] = starts[(long long)q * nch];
+// One workgroup per round: bitonic sort of its entries by (row, column,
+// position) in LDS, then packed[e] = column << kBlkSlotBits | slot of e (in
+// place) and srow[r * kBlkRound + slot] = row.
+constexpr int kBlkSortTpb = 1024;
+__global__ __launch_bounds__(kBlkSortTpb) void blk_round_sort_kernel(
+    const int* __restrict__ round_off, uint32_t* __restrict__ packed,
+    const uint16_t* __restrict__ trow, uint16_t* __restrict__ srow) {
+  __shared__ unsigned long long key[kBlkRound];
+  const int r = blockIdx.x;
+  const int a = round_off[r], n = round_off[r + 1] - a;
+  for (int i = threadIdx.x; i < kBlkRound; i += kBlkSortTpb)
+    key[i] = i < n ? ((unsigned long long)trow[a + i] << 32) |
+                         ((unsigned long long)packed[a + i] << kBlkSlotBits) | (unsigned)i
+                   : ~0ULL;
+  __syncthreads();
+  for (int k = 2; k <= kBlkRound; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < kBlkRound; i += kBlkSortTpb) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const unsigned long long u = key[i], v = key[ixj];
+          const bool up = (i & k) == 0;
+          if ((u > v) == up) {
+            key[i] = v;
+            key[ixj] = u;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < n; k += kBlkSortTpb) {
+    const unsigned long long u = key[k];
+    const int i = (int)(u & (kBlkRound - 1));
+    const uint32_t col = (uint32_t)(u >> kBlkSlotBits) & ((1u << kBlkColBits) - 1);
+    packed[a + i] = (col << kBlkSlotBits) | (uint32_t)k;
+    srow[(long long)r * kBlkRound + k] = (uint16_t)(u >> 32);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Execute kernels
 // ---------------------------------------------------------------------------
 constexpr int kBlkTpb = 512;
-constexpr int kBlkVec = 4;  // 16-byte loads of packed words: 4 entries
-constexpr int kBlkU = 4;    // vector loads per lane per round: 16 entries in flight
+constexpr int kBlkIpt = kBlkRound / kBlkTpb;  // entries per lane per phase (8)
+static_assert(kBlkIpt == 8 && kBlkIpt * kBlkTpb == kBlkRound,
+              "a round is one pass of the workgroup, 8 slots (16 bytes of rows) per lane");
+
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// What a lane reads of one round: its phase-1 entries (wavefront w: entries
+// 512 w .. 512 w + 511, lane l: 64 j + l) and its phase-2 slots 8 t .. 8 t + 7
+// with the rows of the slots before and after them.
+template <typename V>
+struct blk_round_regs {
+  int a, n;
+  uint32_t pw[kBlkIpt];
+  float vv[kBlkIpt];
+  u16x8 rows;
+  int prev_row, next_row;
+};
 
 template <typename V>
-struct blk_vec;
-template <>
-struct blk_vec<float> {
-  __device__ static __forceinline__ f32x4 load(const float* p) {
-    return value_traits<float>::load4(p);
+__device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict__ packed,
+                                               const V* __restrict__ vals,
+                                               const uint16_t* __restrict__ srow,
+                                               const int* __restrict__ round_off, int lane,
+                                               int wave, int k0, blk_round_regs<V>& g) {
+  g.a = round_off[r];
+  g.n = round_off[r + 1] - g.a;
+#pragma unroll
+  for (int j = 0; j < kBlkIpt; ++j) {
+    const int i = wave * 64 * kBlkIpt + j * 64 + lane;
+    g.pw[j] = 0;
+    g.vv[j] = 0.f;
+    if (i < g.n) {
+      g.pw[j] = (uint32_t)ld_idx(reinterpret_cast<const int*>(packed + g.a + i));
+      g.vv[j] = value_traits<V>::load1(vals + g.a + i);
+    }
   }
-};
-template <>
-struct blk_vec<bf16_t> {
-  __device__ static __forceinline__ f32x4 load(const bf16_t* p) {
-    return value_traits<bf16_t>::load4(p);
+  const uint16_t* rs = srow + (long long)r * kBlkRound;
+  g.rows = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+  g.prev_row = -1;
+  g.next_row = -1;
+  if (k0 < g.n) {
+    g.rows = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(rs + k0));
+    if (k0 > 0) g.prev_row = rs[k0 - 1];
+    if (k0 + kBlkIpt < g.n) g.next_row = rs[k0 + kBlkIpt];
   }
-};
-template <>
-struct blk_vec<f16_t> {
-  __device__ static __forceinline__ f32x4 load(const f16_t* p) {
-    return value_traits<f16_t>::load4(p);
-  }
-};
+}
 
-// Workgroup b * S + s: segment s of row block b.  Entries are read in
-// aligned groups of four covering [e0, e1) (the groups may start up to 3
-// entries before e0 and end up to 3 after e1; those lanes add nothing, and
-// the arrays are padded so the reads stay in bounds).
 template <typename V>
 __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     const uint32_t* __restrict__ packed, const V* __restrict__ vals,
-    const int* __restrict__ stream_off, const float* __restrict__ x, int rows, int S,
-    int W, float* __restrict__ partial, long long part_stride) {
+    const uint16_t* __restrict__ srow, const int* __restrict__ stream_round,
+    const int* __restrict__ round_off, const int* __restrict__ blk_start,
+    const int* __restrict__ blk_order, const float* __restrict__ x, int S, int W,
+    float* __restrict__ partial, long long part_stride) {
   __shared__ float acc[kBlkRows];
-  const int q = blockIdx.x;
-  const int b = q / S, s = q - b * S;
-  const int r0 = b << kBlkRowBits;
-  const int nrows = rows - r0 < kBlkRows ? rows - r0 : kBlkRows;
+  __shared__ float stage[kBlkRound];
+  __shared__ float s_v[kBlkTpb / 64];
+  __shared__ int s_f[kBlkTpb / 64];
+  const int s = (int)(blockIdx.x % (unsigned)S);
+  const int b = blk_order[blockIdx.x / (unsigned)S];
+  const int q = b * S + s;  // stream
+  const int r0 = blk_start[b];
+  const int nrows = blk_start[b + 1] - r0;  // <= kBlkRows
   for (int i = threadIdx.x; i < nrows; i += kBlkTpb) acc[i] = 0.f;
-  __syncthreads();
-  const long long e0 = stream_off[q], e1 = stream_off[q + 1];
   const float* __restrict__ xs = x + (long long)s * W;
-  const long long g0 = e0 & ~(long long)(kBlkVec - 1);
-  for (long long base = g0; base < e1; base += (long long)kBlkTpb * kBlkVec * kBlkU) {
-    i32x4 pw[kBlkU];
-    f32x4 vv[kBlkU];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k0 = threadIdx.x * kBlkIpt;
+  const int rb = stream_round[q], re = stream_round[q + 1];
+  blk_round_regs<V> cur;
+  if (rb < re) blk_load_round<V>(rb, packed, vals, srow, round_off, lane, wave, k0, cur);
+  for (int r = rb; r < re; ++r) {
+    const int n = cur.n;
+    // Phase 1: products in column order to their slots; the next round's
+    // loads go out behind this round's x reads.
+    float xv[kBlkIpt];
 #pragma unroll
-    for (int u = 0; u < kBlkU; ++u) {
-      const long long e = base + ((long long)u * kBlkTpb + threadIdx.x) * kBlkVec;
-      if (e < e1) {
-        pw[u] = ld_idx4(reinterpret_cast<const int*>(packed + e));
-        vv[u] = blk_vec<V>::load(vals + e);
-      }
+    for (int j = 0; j < kBlkIpt; ++j) {
+      const int i = wave * 64 * kBlkIpt + j * 64 + lane;
+      xv[j] = i < n ? xs[cur.pw[j] >> kBlkSlotBits] : 0.f;
     }
-    float xv[kBlkU][kBlkVec];
+    blk_round_regs<V> nxt;
+    if (r + 1 < re) blk_load_round<V>(r + 1, packed, vals, srow, round_off, lane, wave, k0, nxt);
 #pragma unroll
-    for (int u = 0; u < kBlkU; ++u) {
-      const long long e = base + ((long long)u * kBlkTpb + threadIdx.x) * kBlkVec;
-#pragma unroll
-      for (int j = 0; j < kBlkVec; ++j) {
-        const bool ok = e + j >= e0 && e + j < e1;
-        xv[u][j] = ok ? xs[(uint32_t)pw[u][j] >> kBlkRowBits] : 0.f;
-      }
+    for (int j = 0; j < kBlkIpt; ++j) {
+      const int i = wave * 64 * kBlkIpt + j * 64 + lane;
+      if (i < n) stage[cur.pw[j] & (kBlkRound - 1)] = cur.vv[j] * xv[j];
     }
+    __syncthreads();
+    // Phase 2: runs of equal rows over slots k0 .. k0 + m - 1.
+    const int m = n - k0 < 0 ? 0 : (n - k0 > kBlkIpt ? kBlkIpt : n - k0);
+    float v = 0.f, first_sum = 0.f;
+    int f = 1, first_row = -1, row = cur.rows[0];
+    bool internal = false;
+    if (m > 0) {
+      v = stage[k0];
 #pragma unroll
-    for (int u = 0; u < kBlkU; ++u) {
-      const long long e = base + ((long long)u * kBlkTpb + threadIdx.x) * kBlkVec;
-#pragma unroll
-      for (int j = 0; j < kBlkVec; ++j) {
-        if (e + j >= e0 && e + j < e1)
-          __hip_atomic_fetch_add(&acc[(uint32_t)pw[u][j] & (kBlkRows - 1)], vv[u][j] * xv[u][j],
-                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (int i = 1; i < kBlkIpt; ++i) {
+        if (i < m) {
+          const float p = stage[k0 + i];
+          const int ri = cur.rows[i];
+          if (ri != row) {
+            if (!internal) {
+              first_sum = v;
+              first_row = row;
+              internal = true;
+            } else {
+              acc[row] += v;  // a run that starts and ends in this lane
+            }
+            row = ri;
+            v = p;
+          } else {
+            v += p;
+          }
+        }
       }
+      f = internal || cur.prev_row != (int)cur.rows[0];
     }
+    const float v_own = v;
+    float pv;
+    int pf;
+    block_seg_scan<kBlkTpb>(v, f, s_v, s_f, pv, pf);
+    // exclusive: the running sum of the previous lane's trailing run
+    const float up = __shfl_up(v, 1, 64);
+    const float ex = lane == 0 ? pv : up;
+    if (m > 0) {
+      const bool cont = k0 > 0 && cur.prev_row == (int)cur.rows[0];
+      const float cin = cont ? ex : 0.f;
+      if (internal) acc[first_row] += cin + first_sum;
+      if (k0 + m >= n || cur.next_row != row) acc[row] += (internal ? 0.f : cin) + v_own;
+    }
+    if (r + 1 < re) cur = nxt;
   }
   __syncthreads();
   float* out = partial + (long long)s * part_stride + r0;
@@ -203,28 +371,29 @@ namespace {
 inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
 template <typename V>
-hipError_t blk_scatter(const csr_args& A, const blk_geom& g, int* cursor, uint32_t* packed,
-                       void* vals, unsigned grid, hipStream_t s) {
+hipError_t blk_scatter(const csr_args& A, const blk_geom& g, int* cursor, uint16_t* trow,
+                       blk_plan_state& P, unsigned grid, hipStream_t s) {
   blk_scatter_kernel<V><<<grid, 256, 0, s>>>(A.offsets, A.indices,
                                              static_cast<const V*>(A.values), g, cursor,
-                                             packed, static_cast<V*>(vals));
+                                             P.packed, trow, static_cast<V*>(P.values));
   return hipGetLastError();
 }
 
 template <typename V>
 hipError_t blk_execute(const blk_plan_state& st, const float* x, hipStream_t s) {
   spmv_blocks_kernel<V><<<(unsigned)((long long)st.nb * st.S), kBlkTpb, 0, s>>>(
-      st.packed, static_cast<const V*>(st.values), st.stream_off, x, st.rows, st.S, st.W,
-      st.partial, st.part_stride);
+      st.packed, static_cast<const V*>(st.values), st.srow, st.stream_round, st.round_off,
+      st.blk_start, st.blk_order, x, st.S, st.W, st.partial, st.part_stride);
   return hipGetLastError();
 }
 }  // namespace
 
-// Segments: a multiple of 8 (one XCD each), enough that W < 2^kBlkColBits.
+// Segments: 4 while 4 segments of 2^kBlkColBits columns hold the columns
+// (each on two XCDs), else a multiple of 8 (one XCD each).
 int blk_segments(long long cols) {
-  const long long per8 = 8LL << kBlkColBits;  // columns 8 segments can hold
-  const long long groups = cols > 0 ? (cols + per8 - 1) / per8 : 1;
-  return (int)(8 * groups);
+  const long long per = 1LL << kBlkColBits;
+  if (cols <= 4 * per) return 4;
+  return (int)(8 * ((cols + 8 * per - 1) / (8 * per)));
 }
 
 hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) {
@@ -236,7 +405,44 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   P.S = blk_segments(A.cols);
   P.W = (int)((A.cols + P.S - 1) / P.S);
   if (P.W < 1) P.W = 1;
-  P.nb = (int)((A.rows + kBlkRows - 1) / kBlkRows);
+  if (A.rows == 0) return hipSuccess;  // nothing to store; execution only records ev1
+  // Row blocks: at most kBlkRows rows and about `target` nonzeros each, a
+  // row closing a block when that overshoots the target less than stopping
+  // short would (rows are never split).  The target aims at two waves of
+  // resident workgroups (two per CU) so that uneven blocks even out.
+  std::vector<int> off_h((size_t)A.rows + 1, 0), starts(1, 0);
+  hipError_t e = hipMemcpyAsync(off_h.data(), A.offsets, ((size_t)A.rows + 1) * sizeof(int),
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  int dev = 0, cus = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  long long want = ((long long)A.rows + kBlkRows - 1) / kBlkRows;
+  const long long by_cus = (2LL * 2 * cus + P.S - 1) / P.S;
+  if (want < by_cus) want = by_cus;
+  const long long target = A.nnz / want > 0 ? A.nnz / want : 1;
+  int rows_in = 0;
+  long long nnz_in = 0;
+  for (int r = 0; r < A.rows; ++r) {
+    const long long L = off_h[r + 1] - off_h[r];
+    if (rows_in > 0 && (rows_in == kBlkRows ||
+                        (nnz_in + L > target && nnz_in + L - target > target - nnz_in))) {
+      starts.push_back(r);
+      rows_in = 0;
+      nnz_in = 0;
+    }
+    ++rows_in;
+    nnz_in += L;
+  }
+  starts.push_back(A.rows);
+  P.nb = (int)starts.size() - 1;
+  // Launch order: heaviest blocks first.
+  std::vector<int> order((size_t)P.nb);
+  for (int b = 0; b < P.nb; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
+    return off_h[starts[u + 1]] - off_h[starts[u]] > off_h[starts[v + 1]] - off_h[starts[v]];
+  });
   const int nch = (P.W + kBlkChunkCols - 1) / kBlkChunkCols;
   const long long nstreams = (long long)P.nb * P.S;
   const long long nbuckets = nstreams * nch;
@@ -244,36 +450,37 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const int vbytes = A.vtype == LOOPS_VALUE_F32 ? 4 : 2;
   P.part_stride = (long long)((A.rows + 63) & ~63);
 
-  // Plan memory: packed words and values (padded by one group of four),
-  // stream offsets, partial outputs.
-  const size_t n_pad = (size_t)A.nnz + kBlkVec;
-  const size_t b_packed = al256(n_pad * sizeof(uint32_t));
-  const size_t b_vals = al256(n_pad * vbytes);
-  const size_t b_streams = al256((size_t)(nstreams + 1) * sizeof(int));
-  const size_t b_part = al256((size_t)P.S * P.part_stride * sizeof(float));
-  hipError_t e = hipMalloc(&P.mem, b_packed + b_vals + b_streams + b_part);
-  if (e != hipSuccess) return e;
-  char* m = static_cast<char*>(P.mem);
-  P.packed = reinterpret_cast<uint32_t*>(m);
-  P.values = m + b_packed;
-  P.stream_off = reinterpret_cast<int*>(m + b_packed + b_vals);
-  P.partial = reinterpret_cast<float*>(m + b_packed + b_vals + b_streams);
-  P.device_bytes = b_packed + b_vals + b_streams + b_part;
-
-  // Build scratch: bucket counts, bucket starts (+1), scan scratch.
+  // Build scratch: bucket counts, bucket starts (+1), rounds per stream and
+  // their exclusive prefix (+1), scan scratch.
   const size_t b_cnt = al256((size_t)nbuckets * sizeof(int));
   const size_t b_start = al256((size_t)(nbuckets + 1) * sizeof(int));
-  const size_t b_scan = al256(exclusive_scan_scratch_bytes(nbuckets));
+  const size_t b_nr = al256((size_t)(nstreams + 1) * sizeof(int));
+  const size_t b_scan = al256(exclusive_scan_scratch_bytes(nbuckets > nstreams ? nbuckets
+                                                                               : nstreams));
+  const size_t b_trow = al256(((size_t)A.nnz + 8) * sizeof(uint16_t));
+  const size_t b_bstart = al256(((size_t)P.nb + 1) * sizeof(int));
+  const size_t b_order = al256(((size_t)P.nb + 1) * sizeof(int));
   char* sc = nullptr;
-  if ((e = hipMalloc(&sc, b_cnt + b_start + b_scan)) != hipSuccess) return e;
+  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart);
+  if (e != hipSuccess) return e;
   int* cnt = reinterpret_cast<int*>(sc);
   int* start = reinterpret_cast<int*>(sc + b_cnt);
-  void* scan_scratch = sc + b_cnt + b_start;
-  blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb};
+  int* nr = reinterpret_cast<int*>(sc + b_cnt + b_start);
+  int* first = reinterpret_cast<int*>(sc + b_cnt + b_start + b_nr);
+  void* scan_scratch = sc + b_cnt + b_start + 2 * b_nr;
+  uint16_t* trow = reinterpret_cast<uint16_t*>(sc + b_cnt + b_start + 2 * b_nr + b_scan);
+  int* bstart = reinterpret_cast<int*>(sc + b_cnt + b_start + 2 * b_nr + b_scan + b_trow);
+  const blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart};
   long long grid = ((long long)A.nnz + 255) / 256;
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
-  e = hipMemsetAsync(P.packed, 0, b_packed + b_vals, s);
+  const unsigned sgrid = (unsigned)((nstreams + 1 + 255) / 256);
+
+  // Plan memory, part 1: stream -> first round (needed to count rounds).
+  const size_t b_sround = al256((size_t)(nstreams + 1) * sizeof(int));
+  int nrounds = 0;
+  e = hipMemcpyAsync(bstart, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice,
+                     s);
   if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, b_cnt, s);
   if (e == hipSuccess && A.nnz > 0) {
     blk_count_kernel<<<(unsigned)grid, 256, 0, s>>>(A.offsets, A.indices, g, cnt);
@@ -281,23 +488,79 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   }
   if (e == hipSuccess) e = launch_exclusive_scan(cnt, nbuckets, start, scan_scratch, s);
   if (e == hipSuccess) {
-    blk_streams_kernel<<<(unsigned)((nstreams + 1 + 255) / 256), 256, 0, s>>>(
-        start, (int)nstreams, nch, P.stream_off);
+    blk_round_count_kernel<<<sgrid, 256, 0, s>>>(start, (int)nstreams, nch, nr);
+    e = hipGetLastError();
+  }
+  // first round of each stream (copied into the plan below)
+  if (e == hipSuccess) e = launch_exclusive_scan(nr, nstreams, first, scan_scratch, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&nrounds, first + nstreams, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(sc);
+    return e;
+  }
+  P.nrounds = nrounds;
+
+  // Plan memory, part 2: entries, rows per slot (kBlkRound per round),
+  // rounds, partials.
+  const size_t n_pad = (size_t)A.nnz + 8;
+  const size_t b_packed = al256(n_pad * sizeof(uint32_t));
+  const size_t b_vals = al256(n_pad * vbytes);
+  const size_t b_srow = al256(((size_t)nrounds * kBlkRound + 8) * sizeof(uint16_t));
+  const size_t b_roff = al256((size_t)(nrounds + 1) * sizeof(int));
+  const size_t b_part = al256((size_t)P.S * P.part_stride * sizeof(float));
+  const size_t total =
+      b_packed + b_vals + b_srow + b_sround + b_roff + b_part + b_bstart + b_order;
+  if ((e = hipMalloc(&P.mem, total)) != hipSuccess) {
+    (void)hipFree(sc);
+    return e;
+  }
+  char* m = static_cast<char*>(P.mem);
+  P.packed = reinterpret_cast<uint32_t*>(m);
+  P.values = m + b_packed;
+  P.srow = reinterpret_cast<uint16_t*>(m + b_packed + b_vals);
+  P.stream_round = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow);
+  P.round_off = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow + b_sround);
+  P.partial = reinterpret_cast<float*>(m + b_packed + b_vals + b_srow + b_sround + b_roff);
+  P.blk_start = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow + b_sround + b_roff +
+                                       b_part);
+  P.blk_order = P.blk_start + b_bstart / sizeof(int);
+  P.device_bytes = total;
+
+  e = hipMemsetAsync(P.mem, 0, b_packed + b_vals + b_srow, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(P.stream_round, first, (size_t)(nstreams + 1) * sizeof(int),
+                       hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(P.blk_start, bstart, ((size_t)P.nb + 1) * sizeof(int),
+                       hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(P.blk_order, order.data(), (size_t)P.nb * sizeof(int),
+                       hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    blk_round_write_kernel<<<sgrid, 256, 0, s>>>(start, (int)nstreams, nch, P.stream_round,
+                                                 A.nnz, P.round_off);
     e = hipGetLastError();
   }
   // the bucket starts become the scatter cursors
   if (e == hipSuccess && A.nnz > 0) {
     switch (A.vtype) {
       case LOOPS_VALUE_F32:
-        e = blk_scatter<float>(A, g, start, P.packed, P.values, (unsigned)grid, s);
+        e = blk_scatter<float>(A, g, start, trow, P, (unsigned)grid, s);
         break;
       case LOOPS_VALUE_BF16:
-        e = blk_scatter<bf16_t>(A, g, start, P.packed, P.values, (unsigned)grid, s);
+        e = blk_scatter<bf16_t>(A, g, start, trow, P, (unsigned)grid, s);
         break;
       case LOOPS_VALUE_F16:
-        e = blk_scatter<f16_t>(A, g, start, P.packed, P.values, (unsigned)grid, s);
+        e = blk_scatter<f16_t>(A, g, start, trow, P, (unsigned)grid, s);
         break;
     }
+  }
+  if (e == hipSuccess && nrounds > 0) {
+    blk_round_sort_kernel<<<(unsigned)nrounds, kBlkSortTpb, 0, s>>>(P.round_off, P.packed,
+                                                                     trow, P.srow);
+    e = hipGetLastError();
   }
   const hipError_t e2 = hipStreamSynchronize(s);
   (void)hipFree(sc);

@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--ipt", type=int, default=0)
     ap.add_argument("--panels", type=int, default=0, help="column panels (0: automatic)")
     ap.add_argument("--layout", default="auto", choices=["auto", "panels", "blocks"],
-                    help="SpMV plan layout (auto: panels)")
+                    help="SpMV plan layout (auto: blocks for this matrix)")
     ap.add_argument("--schedules", default="all",
                     help="comma list of extra schedules to time, 'all' or 'none'")
     ap.add_argument("--cpu-runs", type=int, default=5)
@@ -80,10 +80,13 @@ def copy_calibration(mb, steps):
     print(json.dumps({"copy_calibration_bytes_per_launch": n * 4, "launches": steps}), flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the merge-path kernel, measured by rocprofv3
-    PMC runs and recorded in profiles/ (see DESIGN.md); None if absent."""
-    p = os.path.join(ROOT, "profiles", "merge_path_plan_c2_pmc.md")
+def traffic_from_profile(layout):
+    """HBM bytes per launch of the plan's main kernel on this workload,
+    measured by rocprofv3 PMC runs (FETCH_SIZE, WRITE_SIZE, calibrated
+    against a copy of known size) and recorded in profiles/; None if
+    absent."""
+    name = "spmv_blocks_c2_pmc.md" if layout == "blocks" else "merge_path_plan_c2_pmc.md"
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
@@ -277,7 +280,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(),
+            "traffic": traffic_from_profile(layout),
             "kernel": ("spmv_blocks_kernel" if layout == "blocks" else
                        "spmv_panels_kernel" if n_panels > 1 else "spmv_merge_path_kernel"),
             "kernel_avg_ms": round(kern_avg, 5),

@@ -306,7 +306,9 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
       p->items_per_thread < 0)
     return LOOPS_ERROR_INVALID_VALUE;
   int layout = p->layout;
-  if (layout == LOOPS_PLAN_LAYOUT_AUTO) layout = LOOPS_PLAN_LAYOUT_PANELS;
+  if (layout == LOOPS_PLAN_LAYOUT_AUTO)
+    layout = a.cols >= (1 << 19) && a.rows >= kBlkRows ? LOOPS_PLAN_LAYOUT_BLOCKS
+                                                       : LOOPS_PLAN_LAYOUT_PANELS;
   if (layout == LOOPS_PLAN_LAYOUT_BLOCKS) {
     loops_spmv_plan* pl = new (std::nothrow) loops_spmv_plan();
     if (!pl) return LOOPS_ERROR_ALLOCATION;

@@ -173,10 +173,12 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
  * once (the reference's preprocess_t, merge_path_flat.hxx:150-230) and,
  * with panels > 1, A copied into `panels` column panels whose tiles run on
  * separate XCDs so each XCD's L2 holds only its slice of x.  Deterministic.
- * Layout BLOCKS: A copied into blocks of 16384 rows x column segments (one
- * per XCD), each block's entries bucketed by 64-column chunks; a workgroup
- * accumulates its block's y in LDS, so x is read in column order.  LDS
- * float atomics: results vary in the last bits from run to run.
+ * Layout BLOCKS: A copied into row blocks (at most 15872 rows, about equal
+ * nonzeros) x column segments (4 up to 2^22 columns, else a multiple of 8),
+ * each (block, segment) stream bucketed by 64-column chunks and cut into
+ * rounds of at most 4096 entries sorted by (row, column); a workgroup
+ * accumulates its block's y in LDS, reading x in column order.
+ * Deterministic (no atomics).
  * Neither copied layout has a counterpart in the reference.  Creation
  * allocates and synchronises `stream`; execution allocates nothing and can
  * be captured into a hipGraph.  With PANELS and panels == 1 the plan reads
@@ -185,7 +187,8 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
 typedef struct loops_spmv_plan* loops_spmv_plan_t;
 
 typedef enum loops_plan_layout_t {
-  LOOPS_PLAN_LAYOUT_AUTO = 0,   /* PANELS                                    */
+  LOOPS_PLAN_LAYOUT_AUTO = 0,   /* BLOCKS when cols >= 2^19 and rows >= 15872,
+                                   else PANELS                               */
   LOOPS_PLAN_LAYOUT_PANELS = 1,
   LOOPS_PLAN_LAYOUT_BLOCKS = 2
 } loops_plan_layout_t;

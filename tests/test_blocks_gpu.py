@@ -131,11 +131,11 @@ def test_blocks_graph_replays():
         _check(y.cpu().numpy(), off, idx, val, x_np)
 
 
-def test_blocks_c2_full_size():
+def test_blocks_c2_full_size_auto():
     A_h = synth.c2_powerlaw_2m()
     x_np = synth.uniform(A_h.cols, seed=21)
     A = loops.csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values)
-    plan = loops.spmv_plan(A, layout="blocks")
+    plan = loops.spmv_plan(A)
     assert plan.layout == "blocks" and plan.panels == 4
     y = torch.empty(A_h.rows, device="cuda")
     plan.execute(torch.from_numpy(x_np).cuda(), y)

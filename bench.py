@@ -140,6 +140,7 @@ def main():
 
     n_panels = plans[0].panels
     layout = plans[0].layout
+    n_copies = len(copies)
     nbytes = synth.spmv_bytes(R, C, N, 4)
     flops = 2 * N
 
@@ -272,7 +273,7 @@ def main():
             "zipf": 1.8, "index": "int32", "values": "f32",
             "plan_layout": layout,
             "column_panels" if layout == "panels" else "column_segments": n_panels,
-            "rotated_copies": len(copies), "parallelism": "single-gpu",
+            "rotated_copies": n_copies, "parallelism": "single-gpu",
         },
         "roofline": {
             "bound": "hbm",

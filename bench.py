@@ -245,9 +245,14 @@ def main():
     log(f"cpu: 1 core {cpu_s*1e3:.1f} ms, {mt} threads {cpu_mt_s*1e3:.1f} ms")
 
     if not args.quick and not args.no_forms:
-        # The batched and served forms against plain per-call launches.
+        # bf16 value storage on the same matrix, the SpMM workload, then the
+        # batched and served forms against plain per-call launches.
         from edge_loops_amd import bench_forms
         del copies, plans
+        torch.cuda.empty_cache()
+        extras["bf16_storage_plan"] = bench_forms.spmv_bf16_plan(A_h, x_h, log=log)
+        torch.cuda.empty_cache()
+        extras["spmm_c3"] = bench_forms.spmm_c3(log=log)
         torch.cuda.empty_cache()
         extras["batched"] = bench_forms.batched(log=log)
         torch.cuda.empty_cache()

@@ -219,3 +219,89 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
                                              res["plain_blocking_f32_burst"]["requests_per_s"])
     out["offered_load_per_s"] = lam0
     return out
+
+
+def _f64_product(A_h, values, X):
+    """FP64 product A X (X a vector or a matrix) and sum |a_ij x_j| per row,
+    with scipy.sparse on the host: the error reference of these legs."""
+    import scipy.sparse as sp
+    M = sp.csr_matrix((np.asarray(values, np.float64), A_h.indices, A_h.offsets),
+                      shape=(A_h.rows, A_h.cols))
+    Xd = np.asarray(X, np.float64)
+    return M @ Xd, abs(M) @ np.abs(Xd)
+
+
+def spmv_bf16_plan(A_h, x_h, reps=20, log=print):
+    """The 2M-row matrix's SpMV plan with bf16 value storage (fp32
+    accumulation): step time, GFLOP/s, algorithmic bytes with 2-byte values,
+    and the worst per-row error against an FP64 host product on the stored values
+    relative to sum |a x| (the 1e-5 bar)."""
+    import torch
+    from .algorithms import spmv_plan
+    A = csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values,
+                        value_type="bf16")
+    x = torch.from_numpy(x_h).cuda()
+    y = torch.empty(A_h.rows, device="cuda")
+    p = spmv_plan(A)
+    ms = _time(lambda: p.execute(x, y), reps)
+    torch.cuda.synchronize()
+    stored = A.values.float().cpu().numpy()
+    yg = y.cpu().numpy().astype(np.float64)
+    y64, a = _f64_product(A_h, stored, x_h)
+    rel = np.abs(yg - y64) / np.maximum(a, 1e-30)
+    bad, worst = int(np.sum(np.abs(yg - y64) > 1e-5 * a)), float(rel.max())
+    y32v64, a32 = _f64_product(A_h, A_h.values, x_h)
+    dev = float(np.max(np.abs(yg - y32v64) / np.maximum(a32, 1e-30)))
+    byts = synth.spmv_bytes(A_h.rows, A_h.cols, A_h.nnz, 2)
+    out = {"layout": p.layout, "ms_per_step": ms, "gflops": 2 * A_h.nnz / (ms * 1e-3) / 1e9,
+           "algorithmic_bytes": byts, "gbs": byts / (ms * 1e-3) / 1e9,
+           "rows_beyond_1e-5_vs_fp64_on_stored": int(bad), "worst_rel_err_vs_fp64_on_stored": worst,
+           "worst_rel_dev_vs_fp32_values": dev}
+    log(f"bf16 plan: {ms*1e3:.1f} us/step, {out['gflops']:.1f} GFLOP/s, worst rel err {worst:.2g}")
+    p.destroy()
+    return out
+
+
+def spmm_c3(reps=5, log=print):
+    """SpMM C = A B on the 1M-row power-law matrix (Zipf 2.0, 16M nonzeros)
+    with a dense 1M x 64 fp32 B: group_mapped and merge_path, warm, with the
+    per-row 1e-5 check of a sample of rows against an FP64 host product."""
+    import torch
+    from .algorithms import run_spmm, spmm_workspace_size
+    t0 = time.time()
+    A_h = synth.c3_spmm_1m()
+    k = 64
+    B_h = synth.dense(A_h.cols, k, seed=31)
+    log(f"spmm c3: generated rows={A_h.rows} nnz={A_h.nnz} in {time.time() - t0:.1f}s")
+    A = csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values)
+    B = torch.from_numpy(B_h).cuda()
+    C = torch.empty(A_h.rows, k, device="cuda")
+    byts = synth.spmm_bytes(A_h.rows, A_h.cols, A_h.nnz, k)
+    flops = 2 * A_h.nnz * k
+    out = {"workload": "1M x 1M CSR, Zipf(2.0) rows scaled to 16M nonzeros, B 1M x 64 fp32",
+           "rows": A_h.rows, "nnz": A_h.nnz, "k": k, "algorithmic_bytes": byts}
+    rng = np.random.default_rng(7)
+    sample = np.sort(rng.choice(A_h.rows, 2000, replace=False))
+    for sched in ("group_mapped", "merge_path_flat"):
+        ws = torch.empty(max(spmm_workspace_size(sched, A, k), 256), dtype=torch.uint8,
+                         device="cuda")
+        ms = _time(lambda: run_spmm(sched, A, B, C, workspace=ws), reps)
+        torch.cuda.synchronize()
+        Cs = C.cpu().numpy()[sample]
+        # FP64 host product of the sampled rows only
+        offs = A_h.offsets
+        sub_off = np.zeros(sample.shape[0] + 1, np.int64)
+        lens = offs[sample + 1] - offs[sample]
+        np.cumsum(lens, out=sub_off[1:])
+        idx = np.concatenate([A_h.indices[offs[r]:offs[r + 1]] for r in sample])
+        val = np.concatenate([A_h.values[offs[r]:offs[r + 1]] for r in sample])
+        r64, a64 = _f64_product(synth.HostCSR(sample.shape[0], A_h.cols, sub_off, idx, val), val,
+                                B_h)
+        bad = int(np.sum(np.abs(Cs - r64) > 1e-5 * a64 + 1e-30))
+        out[sched] = {"ms": ms, "gflops": flops / (ms * 1e-3) / 1e9,
+                      "gbs": byts / (ms * 1e-3) / 1e9,
+                      "roofline_frac": byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "sampled_rows_beyond_1e-5": bad}
+        log(f"spmm c3 {sched}: {ms:.3f} ms, {out[sched]['gflops']:.0f} GFLOP/s, "
+            f"{out[sched]['gbs']:.0f} GB/s, bad rows {bad}")
+    return out

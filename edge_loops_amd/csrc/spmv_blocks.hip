@@ -46,6 +46,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "device_common.hpp"
@@ -228,10 +229,33 @@ template <typename V>
 struct blk_round_regs {
   int a, n;
   uint32_t pw[kBlkIpt];
-  float vv[kBlkIpt];
+  uint32_t vw[kBlkIpt];  // raw value words (blk_word_value)
   u16x8 rows;
   int prev_row, next_row;
 };
+
+// Values travel as raw 32-bit words and are converted where they are
+// used, so no load result is waited for inside the guarded load blocks.
+// 16-bit values are read as the aligned 32-bit word holding the pair (the
+// value arrays are padded, so the word stays in bounds); entry e takes the
+// half e & 1.
+__device__ __forceinline__ uint32_t blk_load_word(const float* vals, int e) {
+  return (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const int*>(vals + e));
+}
+template <typename V>
+__device__ __forceinline__ uint32_t blk_load_word(const V* vals, int e) {
+  return (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const int*>(vals + (e & ~1)));
+}
+template <typename V>
+__device__ __forceinline__ float blk_word_value(uint32_t w, int e) {
+  if constexpr (std::is_same<V, float>::value) {
+    return __uint_as_float(w);
+  } else {
+    const uint32_t h = (e & 1) ? (w >> 16) : (w & 0xffffu);
+    if constexpr (std::is_same<V, bf16_t>::value) return __uint_as_float(h << 16);
+    else return static_cast<float>(__builtin_bit_cast(f16_t, (uint16_t)h));
+  }
+}
 
 template <typename V>
 __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict__ packed,
@@ -245,10 +269,10 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
   for (int j = 0; j < kBlkIpt; ++j) {
     const int i = wave * 64 * kBlkIpt + j * 64 + lane;
     g.pw[j] = 0;
-    g.vv[j] = 0.f;
+    g.vw[j] = 0;
     if (i < g.n) {
       g.pw[j] = (uint32_t)ld_idx(reinterpret_cast<const int*>(packed + g.a + i));
-      g.vv[j] = value_traits<V>::load1(vals + g.a + i);
+      g.vw[j] = blk_load_word(vals, g.a + i);
     }
   }
   const uint16_t* rs = srow + (long long)r * kBlkRound;
@@ -300,7 +324,8 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
 #pragma unroll
     for (int j = 0; j < kBlkIpt; ++j) {
       const int i = wave * 64 * kBlkIpt + j * 64 + lane;
-      if (i < n) stage[cur.pw[j] & (kBlkRound - 1)] = cur.vv[j] * xv[j];
+      if (i < n)
+        stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], cur.a + i) * xv[j];
     }
     __syncthreads();
     // Phase 2: runs of equal rows over slots k0 .. k0 + m - 1.

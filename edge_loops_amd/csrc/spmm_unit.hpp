@@ -347,15 +347,26 @@ __device__ __forceinline__ int spmm_fixup_probe(const int* __restrict__ offsets,
   return r;
 }
 
-// spmm_fixup_row: the calling wavefront's lanes stride over the k columns.
+// spmm_fixup_row: the calling wavefront splits its 64 lanes into kp column
+// lanes (kp = k rounded up to a power of two, at most 64) x 64 / kp tile
+// partitions; partition p sums carries t + p, t + p + 64 / kp, ... < t_end
+// for its column, an xor butterfly adds the partitions, and head[t_end] is
+// added last.  The order is fixed by k alone.
 __device__ __forceinline__ void spmm_fixup_row(int k, long long t, long long t_end, int r,
                                                const spmm_ws& ws, float* __restrict__ C,
                                                long long ldc) {
-  for (int c = threadIdx.x & 63; c < k; c += 64) {
-    float s = ws.carry[t * k + c];
-    for (long long v = t + 1; v < t_end; ++v) s += ws.carry[v * k + c];
-    s += ws.head[t_end * k + c];
-    C[(long long)r * ldc + c] = s;
+  const int lane = threadIdx.x & 63;
+  int kp = 1;
+  while (kp < k && kp < 64) kp <<= 1;
+  const int parts = 64 / kp;
+  const int cl = lane & (kp - 1), part = lane / kp;
+  for (int c0 = 0; c0 < k; c0 += kp) {
+    const int c = c0 + cl;
+    float s = 0.f;
+    if (c < k)
+      for (long long v = t + part; v < t_end; v += parts) s += ws.carry[v * k + c];
+    for (int m = kp; m < 64; m <<= 1) s += __shfl_xor(s, m, 64);
+    if (part == 0 && c < k) C[(long long)r * ldc + c] = s + ws.head[t_end * k + c];
   }
 }
 

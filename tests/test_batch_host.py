@@ -87,3 +87,32 @@ def test_committed_table_is_complete():
             assert e["schedule"] in _capi.SCHEDULES
             sched, ipt = t.pick(1000, 5000, 50, 1 if sec == "spmv" else 8)
             assert sched in _capi.SCHEDULES and ipt >= 0
+
+
+def test_table_size_grows_with_items_not_with_their_sizes():
+    # One descriptor and a few prefix counts per item: a batch of large
+    # matrices has the same table as a batch of small ones.
+    small = query([item(1000, 9000, k=16, ipt=8) for _ in range(5)])[1].table_bytes
+    big = query([item(4_000_000, 80_000_000, k=16, ipt=8) for _ in range(5)])[1].table_bytes
+    assert small == big and big < 8192
+    n5 = query([item(1000, 9000) for _ in range(5)])[1].table_bytes
+    n50 = query([item(1000, 9000) for _ in range(50)])[1].table_bytes
+    assert n50 > n5 and n50 < 50 * 1024
+
+
+def test_spmm_variant_mask():
+    # Variant bit = (value type * 2 + (4 columns per lane)) * 3 + (steps 4/8/16 -> 0/1/2).
+    # With the fake addresses B and C rows are 16-byte aligned, so k >= 16 with
+    # ld a multiple of 4 takes 4 columns per lane.
+    def mask(items):
+        st, info = query(items)
+        assert st == _capi.SUCCESS
+        return info.spmm_variant_mask
+    assert mask([item(1000, 9000)]) == 0                                   # SpMV only
+    assert mask([item(1000, 9000, k=8, ipt=8)]) == 1 << 1                  # f32, 1 col, 8
+    assert mask([item(1000, 9000, k=16, ipt=4)]) == 1 << 3                 # f32, 4 cols, 4
+    assert mask([item(1000, 9000, k=16, ipt=16, vt=_capi.VALUE_BF16)]) == 1 << 11
+    assert mask([item(1000, 9000, k=5, ipt=16, vt=_capi.VALUE_F16)]) == 1 << 14
+    assert mask([item(1000, 9000, k=8, ipt=8), item(1000, 9000, k=8, ipt=8),
+                 item(0, 0, k=8, ipt=4)]) == 1 << 1                        # no pieces: no bit
+    assert mask([item(10, 0, k=8, ipt=4)]) == 1 << 0                       # empty rows are work

@@ -92,3 +92,20 @@ def test_python_mirror_surface():
         assert callable(getattr(loops.spmv, name))
     for name in ("thread_mapped", "group_mapped", "merge_path"):
         assert callable(getattr(loops.spmm, name))
+
+
+def test_plan_parameter_validation_without_a_device():
+    # Invalid plan parameters are rejected before anything touches a device.
+    import ctypes
+    base = 1 << 20
+    v = _capi.CsrView(1000, 1000, 9000, base, base + 4096, base + 8192, _capi.VALUE_F32)
+    h = ctypes.c_void_p()
+    L = _capi.lib()
+    for bad in (_capi.PlanParams(0, 0, 0, 3), _capi.PlanParams(0, 0, 0, -1),
+                _capi.PlanParams(0, 0, 9, 1), _capi.PlanParams(-1, 0, 0, 1)):
+        st = L.loops_spmv_plan_create(None, ctypes.byref(v), ctypes.byref(bad), None,
+                                      ctypes.byref(h))
+        assert st == _capi.ERROR_INVALID_VALUE
+    assert L.loops_spmv_plan_layout(None, ctypes.byref(ctypes.c_int32())) == \
+        _capi.ERROR_INVALID_VALUE
+    assert _capi.PLAN_LAYOUTS == {"auto": 0, "panels": 1, "blocks": 2}

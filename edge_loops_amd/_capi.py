@@ -111,6 +111,7 @@ class BatchInfo(ctypes.Structure):
         ("coordinates", ctypes.c_int64),
         ("max_items_per_thread", ctypes.c_int64),
         ("spmm_variant_mask", ctypes.c_int64),
+        ("spmv_class_mask", ctypes.c_int64),
     ]
 
 

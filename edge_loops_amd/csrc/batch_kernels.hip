@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "device_common.hpp"
 #include "loops_internal.hpp"
@@ -45,11 +46,20 @@ namespace loops_mi355x {
 
 enum batch_kind : int { BK_MERGE = 0, BK_ROWS = 1, BK_SPMM = 2, BK_GROUP = 3 };
 
-// SpMM tile variants, one persistent kernel each: value type x columns per
-// lane (1, 4) x merge steps per range (4, 8, 16).
+// Kernel classes, one persistent kernel each, so that every kernel has the
+// registers, LDS and occupancy of the single-matrix kernel for its tile:
+//   SpMV classes 0..2: merge tiles of 4 (and the ROWS / GROUP pieces), 8 and
+//                      16 steps per lane;
+//   SpMM variants:     value type x columns per lane (1, 4) x merge steps
+//                      per range (4, 8, 16), class kSpmvClasses + variant.
+constexpr int kSpmvClasses = 3;
 constexpr int kSpmmVariants = 3 * 2 * 3;
+constexpr int kClasses = kSpmvClasses + kSpmmVariants;
 __host__ __device__ constexpr int spmm_variant(int vtype, int cpl, int ipt) {
   return (vtype * 2 + (cpl == 4 ? 1 : 0)) * 3 + (ipt == 4 ? 0 : ipt == 8 ? 1 : 2);
+}
+__host__ __device__ constexpr int spmv_class(int maxipt) {
+  return maxipt == 4 ? 0 : maxipt == 8 ? 1 : 2;
 }
 
 struct batch_item_dev {
@@ -72,21 +82,20 @@ struct batch_item_dev {
 // Prefix arrays of the table, n_items + 1 entries each: entry i = pieces /
 // slots / coordinates of items 0 .. i-1.
 enum batch_prefix : int {
-  BP_SPMV = 0,   // pieces of the SpMV kernel
-  BP_TILE = 1,   // merge-tile fix-up slots (MERGE items)
-  BP_COORD = 2,  // merge-tile start coordinates (MERGE and SPMM items)
-  BP_UNIT = 3,   // SpMM tile fix-up slots (SPMM items)
-  BP_COUNT = 4
+  BP_TILE = 0,   // merge-tile fix-up slots (MERGE items)
+  BP_COORD = 1,  // merge-tile start coordinates (MERGE and SPMM items)
+  BP_UNIT = 2,   // SpMM tile fix-up slots (SPMM items)
+  BP_COUNT = 3
 };
 
-// SpMM items are also listed grouped by tile variant (spmm_order, stable in
-// item order): variant v owns entries [var_begin[v], var_begin[v + 1]) of
-// spmm_order, and spmm_prefix[j] counts the pieces of entries before j.
+// Items are also listed grouped by kernel class (order, stable in item
+// order): class c owns entries [class_begin[c], class_begin[c + 1]) of
+// order, and cprefix[j] counts the pieces of entries before j.
 struct batch_header {
   long long n_items, n_spmv_work, n_spmm_work, n_tiles, n_coords, n_units;
-  long long items_off, prefix_off, order_off, sprefix_off, n_spmm_items;
-  long long var_begin[kSpmmVariants + 1];
-  unsigned long long* queue;  // work-queue counter sets: SpMV kernel, then each variant
+  long long items_off, prefix_off, order_off, cprefix_off;
+  long long class_begin[kClasses + 1];
+  unsigned long long* queue;  // work-queue counter sets, one per class
 };
 
 __device__ __forceinline__ const long long* batch_prefix_of(const char* table,
@@ -141,7 +150,7 @@ constexpr int kBatchTpb = 256;
 #endif
 constexpr int kQueues = LOOPS_BATCH_QUEUES;
 constexpr int kQueueStride = 32;  // unsigned long longs between counters
-constexpr int kQueueSets = 1 + kSpmmVariants;  // the SpMV kernel, then each variant
+constexpr int kQueueSets = kClasses;
 constexpr size_t kQueueBytes =
     (size_t)kQueueSets * kQueues * kQueueStride * sizeof(unsigned long long);
 
@@ -262,6 +271,40 @@ __global__ __launch_bounds__(256) void batch_prologue_kernel(const char* __restr
 // piece range [lo, hi) of its current item and searches the prefix counts
 // only when a piece falls outside it (the piece index is workgroup-uniform,
 // so every wavefront finds the same item).
+// The pieces of class cls: workgroups claim them through the class's
+// counters and find the item of a piece in the class's prefix counts; fn
+// runs one piece (item, index of the piece within the item).
+template <typename F>
+__device__ __forceinline__ void batch_class_loop(const char* __restrict__ table, int cls,
+                                                 long long* s_next, F&& fn) {
+  const batch_header* h = reinterpret_cast<const batch_header*>(table);
+  const int* order = reinterpret_cast<const int*>(table + h->order_off);
+  const long long jb = h->class_begin[cls], je = h->class_begin[cls + 1];
+  const long long* P = reinterpret_cast<const long long*>(table + h->cprefix_off) + jb;
+  const long long base = P[0], n = P[je - jb] - base;
+  const piece_queue pq(h->queue + (long long)cls * kQueues * kQueueStride);
+  long long i = pq.first(), lo = 0, hi = 0;
+  int item = 0;
+  while (i < n) {
+    long long claim = 0;
+    if (threadIdx.x == 0) claim = pq.claim();
+    const long long g = base + i;
+    if (g < lo || g >= hi) {
+      const int j = batch_wave_find_item(P, je - jb, g);
+      item = order[jb + j];
+      lo = P[j];
+      hi = P[j + 1];
+    }
+    fn(item, (int)(g - lo));
+    if (threadIdx.x == 0) *s_next = claim;
+    __syncthreads();
+    i = *s_next;
+    __syncthreads();  // s_next and the piece's LDS are reused
+  }
+}
+
+// SpMV class spmv_class(MAXIPT): merge tiles of MAXIPT steps per lane (and,
+// in class 0, the ROWS / GROUP pieces).
 template <int MAXIPT>
 __global__ __launch_bounds__(kBatchTpb) void batch_spmv_kernel(const char* __restrict__ table) {
   __shared__ mp_smem<kBatchTpb, MAXIPT> sm;
@@ -269,29 +312,12 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmv_kernel(const char* __res
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const long long* P = batch_prefix_of(table, h, BP_SPMV);
-  const long long n = h->n_spmv_work;
-  const piece_queue pq(h->queue);
-  long long i = pq.first(), lo = 0, hi = 0;
-  int item = 0;
-  while (i < n) {
-    long long claim = 0;
-    if (threadIdx.x == 0) claim = pq.claim();
-    if (i < lo || i >= hi) {
-      item = batch_wave_find_item(P, h->n_items, i);
-      lo = P[item];
-      hi = P[item + 1];
-    }
+  batch_class_loop(table, spmv_class(MAXIPT), &s_next, [&](int item, int sub) {
     const batch_item_dev& it = items[item];
-    const int sub = (int)(i - lo);
     if (it.vtype == LOOPS_VALUE_BF16) run_spmv_piece<bf16_t, MAXIPT>(it, sub, sm);
     else if (it.vtype == LOOPS_VALUE_F16) run_spmv_piece<f16_t, MAXIPT>(it, sub, sm);
     else run_spmv_piece<float, MAXIPT>(it, sub, sm);
-    if (threadIdx.x == 0) s_next = claim;
-    __syncthreads();
-    i = s_next;
-    __syncthreads();  // s_next and the piece's LDS are reused
-  }
+  });
 }
 
 // One instantiation per SpMM tile variant; it works through the pieces of
@@ -309,32 +335,12 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmm_kernel(const char* __res
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const int* order = reinterpret_cast<const int*>(table + h->order_off);
-  const long long jb = h->var_begin[var], je = h->var_begin[var + 1];
-  const long long* P = reinterpret_cast<const long long*>(table + h->sprefix_off) + jb;
-  const long long base = P[0], n = P[je - jb] - base;
-  const piece_queue pq(h->queue + (1 + var) * kQueues * kQueueStride);
-  long long i = pq.first(), lo = 0, hi = 0;
-  int item = 0;
-  while (i < n) {
-    long long claim = 0;
-    if (threadIdx.x == 0) claim = pq.claim();
-    const long long g = base + i;
-    if (g < lo || g >= hi) {
-      const int j = batch_wave_find_item(P, je - jb, g);
-      item = order[jb + j];
-      lo = P[j];
-      hi = P[j + 1];
-    }
+  batch_class_loop(table, kSpmvClasses + var, &s_next, [&](int item, int sub) {
     const batch_item_dev& it = items[item];
     spmm_tile<IPT, CPL, V>(it.offsets, it.indices, static_cast<const V*>(it.values), it.rows,
-                           it.nnz, it.k, it.x, it.ldx, it.y, it.ldy, it.g, (int)(g - lo),
-                           it.coords, it.sx, sm);
-    if (threadIdx.x == 0) s_next = claim;
-    __syncthreads();
-    i = s_next;
-    __syncthreads();
-  }
+                           it.nnz, it.k, it.x, it.ldx, it.y, it.ldy, it.g, sub, it.coords,
+                           it.sx, sm);
+  });
 }
 
 // Merge-tile fix-up across the batch: global slot s belongs to the item
@@ -440,7 +446,7 @@ inline spmm_pick spmm_shape_for(const loops_batch_item_t& b) {
 namespace {
 // What one item contributes to a batch.
 struct item_counts {
-  int kind = BK_MERGE, ipt = 0, g = 0, cpl = 0, ti = 0, variant = -1;
+  int kind = BK_MERGE, ipt = 0, g = 0, cpl = 0, ti = 0, cls = 0;
   long long spmv_work = 0, spmm_work = 0, tiles = 0, coords = 0, units = 0;
   size_t spmm_ws = 0;
 };
@@ -477,6 +483,7 @@ loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
     c->ipt = b.items_per_thread ? b.items_per_thread : 8;
     if (c->ipt != 4 && c->ipt != 8 && c->ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
     c->ti = kBatchTpb * c->ipt;
+    c->cls = spmv_class(c->ipt);
     const long long T = (total + c->ti - 1) / c->ti;
     c->spmv_work = T;
     c->tiles = T;
@@ -495,7 +502,7 @@ loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
     c->units = U;
     c->coords = U + 1;
     c->spmm_ws = U > 0 ? a256(spmm_ws_bytes_impl(U, (int)b.k)) : 0;
-    c->variant = spmm_variant(A.value_type, c->cpl, c->ipt);
+    c->cls = kSpmvClasses + spmm_variant(A.value_type, c->cpl, c->ipt);
   }
   return LOOPS_SUCCESS;
 }
@@ -511,18 +518,16 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
   long long n_spmv_work = 0, n_spmm_work = 0, n_tiles = 0, n_coords = 0, n_units = 0;
   int max_ipt = 0;
   size_t spmm_ws_bytes = 0;
-  long long var_count[kSpmmVariants] = {};
-  long long n_spmm_items = 0;
-  int64_t var_mask = 0;
+  long long cls_count[kClasses] = {};
+  int64_t var_mask = 0, spmv_mask = 0;
   item_counts c;
   for (int64_t i = 0; i < n; ++i) {
     const loops_status_t st = count_item(items[i], &c);
     if (st != LOOPS_SUCCESS) return st;
-    if (c.kind == BK_SPMM) {
-      ++var_count[c.variant];
-      ++n_spmm_items;
-      if (c.spmm_work > 0) var_mask |= (int64_t)1 << c.variant;
-    }
+    ++cls_count[c.cls];
+    if (c.kind == BK_SPMM && c.spmm_work > 0)
+      var_mask |= (int64_t)1 << (c.cls - kSpmvClasses);
+    if (c.kind != BK_SPMM && c.spmv_work > 0) spmv_mask |= (int64_t)1 << c.cls;
     n_spmv_work += c.spmv_work;
     n_spmm_work += c.spmm_work;
     n_tiles += c.tiles;
@@ -544,15 +549,14 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
   hd.n_tiles = n_tiles;
   hd.n_coords = n_coords;
   hd.n_units = n_units;
-  hd.n_spmm_items = n_spmm_items;
-  hd.var_begin[0] = 0;
-  for (int v = 0; v < kSpmmVariants; ++v) hd.var_begin[v + 1] = hd.var_begin[v] + var_count[v];
+  hd.class_begin[0] = 0;
+  for (int q = 0; q < kClasses; ++q) hd.class_begin[q + 1] = hd.class_begin[q] + cls_count[q];
   hd.items_off = a256(sizeof(batch_header));
   hd.prefix_off = hd.items_off + a256((size_t)n * sizeof(batch_item_dev));
   hd.order_off = hd.prefix_off + a256((size_t)BP_COUNT * (n + 1) * sizeof(long long));
-  hd.sprefix_off = hd.order_off + a256((size_t)n_spmm_items * sizeof(int));
+  hd.cprefix_off = hd.order_off + a256((size_t)n * sizeof(int));
   info->table_bytes =
-      (int64_t)(hd.sprefix_off + a256((size_t)(n_spmm_items + 1) * sizeof(long long)));
+      (int64_t)(hd.cprefix_off + a256((size_t)(n + 1) * sizeof(long long)));
   info->workspace_bytes = (int64_t)(kQueueBytes + tile_ws + coord_ws + spmm_ws_bytes + 256);
   info->work_items = n_spmv_work + n_spmm_work;
   info->spmv_tiles = n_tiles;
@@ -561,6 +565,7 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
   info->coordinates = n_coords;
   info->max_items_per_thread = max_ipt;
   info->spmm_variant_mask = var_mask;
+  info->spmv_class_mask = spmv_mask;
   if (!out_table) return LOOPS_SUCCESS;
 
   // Second pass: fill the table.
@@ -573,9 +578,10 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
   long long* P[BP_COUNT];
   for (int q = 0; q < BP_COUNT; ++q) P[q] = pre + q * (n + 1);
   int* order = reinterpret_cast<int*>(table + hd.order_off);
-  long long* SP = reinterpret_cast<long long*>(table + hd.sprefix_off);
-  long long var_next[kSpmmVariants];
-  for (int v = 0; v < kSpmmVariants; ++v) var_next[v] = hd.var_begin[v];
+  long long* CP = reinterpret_cast<long long*>(table + hd.cprefix_off);
+  long long cls_next[kClasses];
+  for (int q = 0; q < kClasses; ++q) cls_next[q] = hd.class_begin[q];
+  std::vector<long long> pieces((size_t)n);
   fixup_ws all_tiles = n_tiles > 0 ? carve_fixup_ws(dev_ws, n_tiles) : fixup_ws{};
   loops_coord_t* all_coords = reinterpret_cast<loops_coord_t*>(dev_ws + tile_ws);
   char* spmm_base = dev_ws + tile_ws + coord_ws;
@@ -583,7 +589,8 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
   for (int64_t i = 0; i < n; ++i) {
     const loops_batch_item_t& b = items[i];
     (void)count_item(b, &c);  // validated in the first pass
-    P[BP_SPMV][i] = wv;
+    order[cls_next[c.cls]++] = (int)i;
+    pieces[i] = c.kind == BK_SPMM ? c.spmm_work : c.spmv_work;
     P[BP_TILE][i] = slot;
     P[BP_COORD][i] = coord;
     P[BP_UNIT][i] = unit;
@@ -615,7 +622,6 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
     }
     if (c.coords > 0) d.coords = all_coords + coord;
     if (c.kind == BK_SPMM) {
-      order[var_next[c.variant]++] = (int)i;
       d.units = (int)c.units;
       if (c.units > 0) {
         d.sx = carve_spmm_ws(spmm_base, c.units, (int)b.k);
@@ -629,14 +635,14 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
     unit += c.units;
     its[i] = d;
   }
-  P[BP_SPMV][n] = wv;
-  // SpMM pieces in variant order: each variant's pieces are contiguous.
-  long long sp = 0;
-  for (long long j = 0; j < n_spmm_items; ++j) {
-    SP[j] = sp;
-    sp += its[order[j]].units;
+  // Pieces in class order: each class's pieces are contiguous.
+  long long cp = 0;
+  for (long long j = 0; j < n; ++j) {
+    CP[j] = cp;
+    cp += pieces[order[j]];
   }
-  SP[n_spmm_items] = sp;
+  CP[n] = cp;
+  (void)wv;
   (void)wm;
   P[BP_TILE][n] = slot;
   P[BP_COORD][n] = coord;
@@ -730,12 +736,12 @@ hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (spmv_work > 0) {
-    const long long mi = info->max_items_per_thread;
-    long long g = grid > 0 ? grid : (mi > 8 ? rg.spmv16 : mi > 4 ? rg.spmv8 : rg.spmv4);
+  for (int q = 0; q < kSpmvClasses && spmv_work > 0; ++q) {
+    if (!((info->spmv_class_mask >> q) & 1)) continue;
+    long long g = grid > 0 ? grid : (q == 2 ? rg.spmv16 : q == 1 ? rg.spmv8 : rg.spmv4);
     if (g > spmv_work) g = spmv_work;
-    if (mi > 8) batch_spmv_kernel<16><<<(unsigned)g, kBatchTpb, 0, s>>>(t);
-    else if (mi > 4) batch_spmv_kernel<8><<<(unsigned)g, kBatchTpb, 0, s>>>(t);
+    if (q == 2) batch_spmv_kernel<16><<<(unsigned)g, kBatchTpb, 0, s>>>(t);
+    else if (q == 1) batch_spmv_kernel<8><<<(unsigned)g, kBatchTpb, 0, s>>>(t);
     else batch_spmv_kernel<4><<<(unsigned)g, kBatchTpb, 0, s>>>(t);
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -15,7 +15,8 @@ candidate (items repeated until the batch holds enough work to fill the
 GPU), captured in a graph and replayed; the fastest candidate wins.  The
 table is plain JSON: bucket key -> {"schedule", "tile", "us": {candidate:
 microseconds}}.  Lookup falls back to the nearest tuned bucket, then to
-merge_path_flat with 8 steps per lane (SpMM: 8 steps per range).
+merge_path_flat with 8 steps per lane (SpMM: 8 steps per range); a bucket's
+measured winner is used only when it beats that default by PICK_MARGIN.
 """
 from __future__ import annotations
 
@@ -29,6 +30,10 @@ SPMV_MERGE = [4, 8, 16]
 SPMM_IPT = [4, 8, 16]
 DEFAULT_SPMV = ("merge_path_flat", 8)
 DEFAULT_SPMM = ("merge_path_flat", 8)
+# A measured pick replaces the default only when it is at least this much
+# faster: smaller differences are within run-to-run noise, and in a mixed
+# batch every extra tile class costs a persistent kernel of its own.
+PICK_MARGIN = 0.05
 TABLE_PATH = os.path.join(os.path.dirname(__file__), "tuning", "mi355x_batch.json")
 
 
@@ -112,11 +117,17 @@ class Table:
             e = tab[near]
         else:
             return DEFAULT_SPMV if k == 1 else DEFAULT_SPMM
+        default = DEFAULT_SPMV if k == 1 else DEFAULT_SPMM
         # JSON keeps tiles as strings ("8", "x4"); numeric ones become int.
         t = e["tile"]
         t = int(t) if str(t).isdigit() else t
         if k > 1 and t not in SPMM_IPT:
-            return DEFAULT_SPMM  # an entry from another tile design
+            return default  # an entry from another tile design
+        us = e.get("us", {})
+        best = us.get(f"{e['schedule']}:{e['tile']}")
+        base = us.get(f"{default[0]}:{default[1]}")
+        if best is not None and base is not None and best > (1.0 - PICK_MARGIN) * base:
+            return default
         return e["schedule"], t
 
 

@@ -260,6 +260,9 @@ typedef struct loops_batch_info_t {
   int64_t max_items_per_thread; /* largest merge tile of the SpMV items   */
   int64_t spmm_variant_mask; /* bit v: SpMM pieces of tile variant v exist;
                                 one persistent kernel is launched per bit  */
+  int64_t spmv_class_mask;   /* bit c: SpMV pieces of class c exist (0: merge
+                                tiles of 4 steps, thread / group mapped; 1:
+                                8 steps; 2: 16 steps); one kernel per bit   */
 } loops_batch_info_t;
 
 loops_status_t loops_batch_query(const loops_batch_item_t* items, int64_t n,

@@ -116,3 +116,17 @@ def test_spmm_variant_mask():
     assert mask([item(1000, 9000, k=8, ipt=8), item(1000, 9000, k=8, ipt=8),
                  item(0, 0, k=8, ipt=4)]) == 1 << 1                        # no pieces: no bit
     assert mask([item(10, 0, k=8, ipt=4)]) == 1 << 0                       # empty rows are work
+
+
+def test_pick_margin_keeps_the_default_for_small_gains():
+    t = tune.Table()
+    key = tune.features(1000, 12000, 400, 1)
+    us = {"merge_path_flat:4": 101.0, "merge_path_flat:8": 100.0, "merge_path_flat:16": 97.0}
+    t.data["spmv"][key] = {"schedule": "merge_path_flat", "tile": "16", "us": us}
+    assert t.pick(1000, 12000, 400) == tune.DEFAULT_SPMV          # 3 % < margin
+    us["merge_path_flat:16"] = 90.0
+    assert t.pick(1000, 12000, 400) == ("merge_path_flat", 16)    # 10 % > margin
+    kkey = tune.features(1000, 12000, 400, 16)
+    t.data["spmm"][kkey] = {"schedule": "merge_path_flat", "tile": "4",
+                            "us": {"merge_path_flat:4": 99.0, "merge_path_flat:8": 100.0}}
+    assert t.pick(1000, 12000, 400, 16) == tune.DEFAULT_SPMM

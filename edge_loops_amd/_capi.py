@@ -121,6 +121,7 @@ PLAN_LAYOUTS = {"auto": 0, "panels": 1, "blocks": 2}
 # loops_server_mode_t
 SERVER_BATCHED = 0
 SERVER_PLAIN = 1
+SERVER_FLAG_REFERENCE_SCHEDULES = 1
 
 
 class ServerConfig(ctypes.Structure):

@@ -168,25 +168,37 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
            "pool_rows": int(sum(m.rows for m in mats)), "requests": n_requests,
            "mean_k": float(ks.mean()), "slots": slots}
 
-    def server(mode, vt, plain_slots=64):
+    def server(mode, vt, plain_slots=64, reference=False, warm=200):
         srv = Server(mode=mode, arena_bytes=32 << 30, scratch_bytes=1 << 30,
-                     slots=slots if mode == "batched" else plain_slots)
+                     slots=slots if mode == "batched" else plain_slots,
+                     reference_schedules=reference)
         dev, tun = pools[vt]
         ids = [srv.add_matrix(A, tunings=t) for A, t in zip(dev, tun)]
         reqs = [(ids[m], x1[m] if k == 1 else X32[m][:, :k]) for m, k in zip(mid, ks)]
-        srv.replay(reqs[:200], np.zeros(200))  # warm-up
+        srv.replay(reqs[:warm], np.zeros(warm))  # warm-up
         return srv, reqs
 
-    def run(srv, reqs, rate):
+    def run(srv, reqs, rate, work=None):
         arr = np.zeros(len(reqs)) if rate is None else poisson_arrivals(len(reqs), rate, seed + 1)
         lat, span = srv.replay(reqs, arr)
         r = _pcts(lat)
         r["requests_per_s"] = len(reqs) / span
-        r["gflops"] = flops / span / 1e9
+        r["gflops"] = (flops if work is None else work) / span / 1e9
         r["offered_per_s"] = rate
         return r
 
     res = {}
+    # Per-call launches of the reference's own entry points (merge_path_flat
+    # SpMV, thread_mapped SpMM -- its only SpMM): burst capacity on the first
+    # n_ref requests (thread_mapped walks each long row with one lane).
+    n_ref = min(300, n_requests)
+    srv, reqs = server("plain", "f32", reference=True, warm=20)
+    work_ref = float(sum(2 * mats[m].nnz * k for m, k in zip(mid[:n_ref], ks[:n_ref])))
+    res["plain_reference_f32_burst"] = run(srv, reqs[:n_ref], None, work_ref)
+    res["plain_reference_f32_burst"]["requests"] = n_ref
+    srv.close()
+    log(f"plain (reference entry points): capacity "
+        f"{res['plain_reference_f32_burst']['requests_per_s']:.0f} req/s")
     # Blocking per-call: one request on the GPU at a time, as with the
     # reference's host entry points, which synchronise before returning.
     srv, reqs = server("plain", "f32", plain_slots=1)
@@ -217,6 +229,9 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
     out["capacity_ratio_f32"] = res["batched_f32_burst"]["requests_per_s"] / cap_plain
     out["capacity_ratio_f32_vs_blocking"] = (res["batched_f32_burst"]["requests_per_s"] /
                                              res["plain_blocking_f32_burst"]["requests_per_s"])
+    out["capacity_ratio_f32_vs_reference_entry_points"] = (
+        res["batched_f32_burst"]["requests_per_s"] /
+        res["plain_reference_f32_burst"]["requests_per_s"])
     out["offered_load_per_s"] = lam0
     return out
 

@@ -167,6 +167,9 @@ struct loops_server {
     return std::chrono::duration<double>(steady::now() - t0).count();
   }
   bool plain() const { return cfg.mode == LOOPS_SERVER_PLAIN; }
+  bool reference_schedules() const {
+    return (cfg.flags & LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES) != 0;
+  }
 };
 
 namespace {
@@ -193,12 +196,18 @@ loops_status_t launch(loops_server* S, slot_t& sl, const std::vector<request_t*>
     loops_params_t p{};
     loops_status_t st;
     if (r->k == 1) {
-      if (r->schedule == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
+      loops_schedule_t sched = r->schedule;
+      if (S->reference_schedules()) {
+        sched = LOOPS_SCHEDULE_MERGE_PATH_FLAT;  // library defaults
+      } else if (sched == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
         p.threads_per_block = 256;
         p.items_per_thread = r->ipt ? r->ipt : 8;
       }
-      st = loops_spmv(S->handle, r->schedule, &r->m.A, r->x, r->y, &p, S->plain_ws,
+      st = loops_spmv(S->handle, sched, &r->m.A, r->x, r->y, &p, S->plain_ws,
                       (size_t)S->cfg.scratch_bytes, sl.stream);
+    } else if (S->reference_schedules()) {
+      st = loops_spmm(S->handle, LOOPS_SCHEDULE_THREAD_MAPPED, &r->m.A, r->k, r->x, r->ldx,
+                      r->y, r->k, &p, S->plain_ws, (size_t)S->cfg.scratch_bytes, sl.stream);
     } else {
       p.items_per_thread = r->ipt;
       st = loops_spmm(S->handle, LOOPS_SCHEDULE_MERGE_PATH_FLAT, &r->m.A, r->k, r->x,
@@ -367,7 +376,9 @@ loops_status_t loops_server_create(const loops_server_config_t* config, loops_se
   *out = nullptr;
   loops_server_config_t c = config ? *config : loops_server_config_t{};
   if (c.arena_bytes < 0 || c.scratch_bytes < 0 || c.slots < 0 || c.max_batch_requests < 0 ||
-      (c.mode != LOOPS_SERVER_BATCHED && c.mode != LOOPS_SERVER_PLAIN) || c.flags != 0)
+      (c.mode != LOOPS_SERVER_BATCHED && c.mode != LOOPS_SERVER_PLAIN) ||
+      (c.flags & ~LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES) != 0 ||
+      (c.flags != 0 && c.mode != LOOPS_SERVER_PLAIN))
     return LOOPS_ERROR_INVALID_VALUE;
   if (c.arena_bytes == 0) c.arena_bytes = (int64_t)1 << 30;
   if (c.scratch_bytes == 0) c.scratch_bytes = (int64_t)64 << 20;
@@ -479,11 +490,17 @@ loops_status_t loops_server_submit(loops_server_t S, int32_t matrix, int64_t k, 
     size_t need = 0;
     loops_params_t p{};
     if (k == 1) {
-      if (r->schedule == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
+      loops_schedule_t sched = r->schedule;
+      if (S->reference_schedules()) {
+        sched = LOOPS_SCHEDULE_MERGE_PATH_FLAT;
+      } else if (sched == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
         p.threads_per_block = 256;
         p.items_per_thread = r->ipt ? r->ipt : 8;
       }
-      st = loops_spmv_workspace_size(nullptr, r->schedule, &r->m.A, &p, &need);
+      st = loops_spmv_workspace_size(nullptr, sched, &r->m.A, &p, &need);
+    } else if (S->reference_schedules()) {
+      st = loops_spmm_workspace_size(nullptr, LOOPS_SCHEDULE_THREAD_MAPPED, &r->m.A, k, &p,
+                                     &need);
     } else {
       p.items_per_thread = r->ipt;
       st = loops_spmm_workspace_size(nullptr, LOOPS_SCHEDULE_MERGE_PATH_FLAT, &r->m.A, k, &p,

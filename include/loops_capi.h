@@ -293,6 +293,12 @@ typedef enum loops_server_mode_t {
   LOOPS_SERVER_PLAIN = 1
 } loops_server_mode_t;
 
+/* PLAIN mode only: launch every request through the reference's own entry
+ * points -- spmv::merge_path_flat (default tiles) for k == 1 and
+ * spmm::thread_mapped (algorithms/spmm/thread_mapped.cuh:67-71, its only
+ * SpMM) for k > 1 -- instead of the tuned merge-path kernels. */
+#define LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES 1u
+
 typedef struct loops_server_config_t {
   int64_t arena_bytes;        /* outputs of admitted requests; 0 = 1 GiB        */
   int64_t scratch_bytes;      /* per slot: batch table + workspace; 0 = 64 MiB  */
@@ -300,7 +306,7 @@ typedef struct loops_server_config_t {
                                  in flight (PLAIN, 0 = 64)                      */
   int32_t max_batch_requests; /* 0 = 512                                        */
   int32_t mode;               /* loops_server_mode_t                            */
-  uint32_t flags;             /* reserved, 0                                    */
+  uint32_t flags;             /* LOOPS_SERVER_FLAG_*, or 0                       */
 } loops_server_config_t;
 
 /* A resident matrix (device CSR, owned by the caller, kept valid while the

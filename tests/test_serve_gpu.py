@@ -90,6 +90,23 @@ def test_plain_mode_equals_batched():
         assert np.array_equal(a, b)
 
 
+def test_plain_reference_schedules():
+    # Plain mode through the reference's own entry points: k == 1 via
+    # merge_path_flat (default tiles), k > 1 via thread_mapped SpMM, which
+    # sums each row in CSR order (bit-identical to the fp32 oracle).
+    with Server(mode="plain", reference_schedules=True) as srv:
+        dev, reqs = _setup(srv, "f32")
+        tickets = [srv.submit(mid, x) for mid, m, x, xh in reqs]
+        outs = [srv.result(t).cpu().numpy() for t in tickets]
+    for (mid, m, x, xh), y in zip(reqs, outs):
+        _check_oracle(dev, m, xh, y)
+        if xh.ndim == 2:
+            off, idx, val, _ = MATS[m]
+            assert np.array_equal(y, oracle.spmm_f32(off, idx, val, xh))
+    with pytest.raises(loops.LoopsError):
+        Server(mode="batched", reference_schedules=True)
+
+
 def test_memory_admission_small_arena():
     biggest = max((off.shape[0] - 1) * k for off, *_ in MATS for k in KS) * 4
     arena = 2 * ((biggest + 255) // 256 * 256)

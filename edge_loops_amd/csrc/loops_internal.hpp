@@ -112,6 +112,13 @@ struct blk_plan_state {
   int* blk_order = nullptr;     // blocks in launch order (heaviest first)
   float* partial = nullptr;     // S arrays of part_stride floats
   long long part_stride = 0;
+  // Rows longer than a quarter of a block's share are cut into pieces
+  // (entry i of a row of m pieces to piece i % m).  Blocks, the LDS
+  // accumulator and the partials count pieces ("virtual rows"); vrow[r] is
+  // the first virtual row of row r (rows + 1 entries), null when no row is
+  // cut (virtual row = row).
+  int nsplit = 0, vrows = 0;
+  int* vrow = nullptr;
   void* mem = nullptr;
   size_t device_bytes = 0;
 };

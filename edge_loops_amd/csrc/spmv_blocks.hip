@@ -8,8 +8,11 @@
 //
 //   rows    are cut into blocks of at most kBlkRows rows and about equal
 //           nonzero counts (enough blocks for two waves of resident
-//           workgroups), launched heaviest first; a workgroup keeps the y
-//           of its block in LDS (kBlkRows floats), two workgroups per CU;
+//           workgroups, less a few), launched heaviest first; a workgroup
+//           keeps the y of its block in LDS (kBlkRows floats), two
+//           workgroups per CU.  A row longer than a quarter of a block's
+//           share is cut into m pieces (entry i to piece i % m) that count
+//           as rows of their own;
 //   columns are cut into S segments of W columns (S = 4 when 4 segments of
 //           2^kBlkColBits columns hold them, else a multiple of 8), and
 //           workgroup b * S + s takes segment s of block b, so a segment
@@ -30,7 +33,8 @@
 // one writer per row per round, so no atomics, and every sum has an order
 // fixed by the plan: results are bitwise reproducible.  Every workgroup
 // writes its block's partial y for its segment; a combine kernel adds the S
-// partials of each row in segment order.
+// partials of each row in segment order (and a cut row's pieces in piece
+// order).
 //
 // Bytes per fp32 entry: 4 (value) + 4 (column | slot) + 2 (row of a slot).
 // The loads of round r + 1 are issued while round r's x reads are in
@@ -59,10 +63,20 @@ namespace loops_mi355x {
 // ---------------------------------------------------------------------------
 struct blk_geom {
   int rows, cols, nnz, S, W, nch, nb;  // nch: chunks per segment; nb: row blocks
-  const int* blk_start;                // first row of each block (nb + 1)
+  const int* blk_start;                // first virtual row of each block (nb + 1)
+  const int* vrow;                     // first virtual row of each row, or null
 };
 
-// Block holding row r: the largest b with blk_start[b] <= r.
+// Virtual row of entry nz of row r: the row itself, or for a row cut into
+// m pieces piece (nz - row start) % m.
+__device__ __forceinline__ int blk_vrow_of(const blk_geom& g, const int* __restrict__ offsets,
+                                           int r, long long nz) {
+  if (!g.vrow) return r;
+  const int v = g.vrow[r], m = g.vrow[r + 1] - v;
+  return m > 1 ? v + (int)((nz - offsets[r]) % m) : v;
+}
+
+// Block holding virtual row r: the largest b with blk_start[b] <= r.
 __device__ __forceinline__ int blk_block_of(const blk_geom& g, int r) {
   int lo = 0, hi = g.nb;
   while (hi - lo > 1) {
@@ -96,7 +110,7 @@ __global__ __launch_bounds__(256) void blk_count_kernel(const int* __restrict__ 
                                                         blk_geom g, int* __restrict__ cnt) {
   for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
        nz += (long long)gridDim.x * blockDim.x) {
-    const int r = blk_row_of(offsets, g.rows, nz);
+    const int r = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
     atomicAdd(&cnt[blk_bucket(g, blk_block_of(g, r), indices[nz])], 1);
   }
 }
@@ -162,7 +176,7 @@ __global__ __launch_bounds__(256) void blk_scatter_kernel(
     uint32_t* __restrict__ packed, uint16_t* __restrict__ trow, V* __restrict__ vals) {
   for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
        nz += (long long)gridDim.x * blockDim.x) {
-    const int r = blk_row_of(offsets, g.rows, nz);
+    const int r = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
     const int b = blk_block_of(g, r);
     const int col = indices[nz];
     const int pos = atomicAdd(&cursor[blk_bucket(g, b, col)], 1);
@@ -377,14 +391,33 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
   for (int i = threadIdx.x; i < nrows; i += kBlkTpb) out[i] = acc[i];
 }
 
-// y[r] = ((partial_0[r] + partial_1[r]) + ...) + partial_{S-1}[r]
+// Sum of the S partials of virtual row v in segment order, loads four at a
+// time (S is 4 or a multiple of 8).
+__device__ __forceinline__ float blk_partials(const float* __restrict__ partial,
+                                              long long part_stride, int S, long long v) {
+  float sum = 0.f;
+  for (int s0 = 0; s0 < S; s0 += 4) {
+    float g[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g[u] = partial[(long long)(s0 + u) * part_stride + v];
+    sum = s0 == 0 ? ((g[0] + g[1]) + g[2]) + g[3] : (((sum + g[0]) + g[1]) + g[2]) + g[3];
+  }
+  return sum;
+}
+
+// y[r] = ((partial_0[r] + partial_1[r]) + ...) + partial_{S-1}[r]; a row cut
+// into pieces adds its pieces' sums in piece order.
 __global__ __launch_bounds__(256) void spmv_blocks_combine_kernel(
     const float* __restrict__ partial, long long part_stride, int S, int rows,
-    float* __restrict__ y) {
-  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
-       r += (long long)gridDim.x * blockDim.x) {
-    float sum = partial[r];
-    for (int s = 1; s < S; ++s) sum += partial[(long long)s * part_stride + r];
+    const int* __restrict__ vrow, float* __restrict__ y) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+    if (!vrow) {
+      y[r] = blk_partials(partial, part_stride, S, r);
+      continue;
+    }
+    const int v0 = vrow[r], v1 = vrow[r + 1];
+    float sum = blk_partials(partial, part_stride, S, v0);
+    for (int v = v0 + 1; v < v1; ++v) sum += blk_partials(partial, part_stride, S, v);
     y[r] = sum;
   }
 }
@@ -431,10 +464,18 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   P.W = (int)((A.cols + P.S - 1) / P.S);
   if (P.W < 1) P.W = 1;
   if (A.rows == 0) return hipSuccess;  // nothing to store; execution only records ev1
-  // Row blocks: at most kBlkRows rows and about `target` nonzeros each, a
-  // row closing a block when that overshoots the target less than stopping
-  // short would (rows are never split).  The target aims at two waves of
-  // resident workgroups (two per CU) so that uneven blocks even out.
+  // Row blocks: at most kBlkRows (virtual) rows and about `target` nonzeros
+  // each, a row closing a block when that overshoots the target less than
+  // stopping short would.  The target aims at two waves of resident
+  // workgroups (two per CU) less 1/32, so that uneven blocks even out and
+  // every XCD keeps a few slots spare (with exactly two waves the dispatcher
+  // left a few workgroups of some XCDs to start a third, 45 us late, on the
+  // 2M-row power-law matrix); it grows until the cut yields no more blocks
+  // than that.  A row longer than a quarter of the target (and
+  // than a round) is cut into pieces, entry i of m to piece i % m, so that
+  // every piece spans the columns as the row does and no block's streams
+  // outgrow the others' (the heaviest blocks run first, the lightest fill
+  // in behind them).
   std::vector<int> off_h((size_t)A.rows + 1, 0), starts(1, 0);
   hipError_t e = hipMemcpyAsync(off_h.data(), A.offsets, ((size_t)A.rows + 1) * sizeof(int),
                                 hipMemcpyDeviceToHost, s);
@@ -444,36 +485,71 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   long long want = ((long long)A.rows + kBlkRows - 1) / kBlkRows;
-  const long long by_cus = (2LL * 2 * cus + P.S - 1) / P.S;
+  const long long by_cus = (2LL * 2 * cus * 31 / 32) / P.S;
   if (want < by_cus) want = by_cus;
-  const long long target = A.nnz / want > 0 ? A.nnz / want : 1;
-  int rows_in = 0;
-  long long nnz_in = 0;
+  long long target = A.nnz / want > 0 ? A.nnz / want : 1;
+  const long long cap = target / 4 > kBlkRound ? target / 4 : kBlkRound;
+  std::vector<int> vrow_h;
+  long long V = A.rows;
   for (int r = 0; r < A.rows; ++r) {
     const long long L = off_h[r + 1] - off_h[r];
-    if (rows_in > 0 && (rows_in == kBlkRows ||
-                        (nnz_in + L > target && nnz_in + L - target > target - nnz_in))) {
-      starts.push_back(r);
-      rows_in = 0;
-      nnz_in = 0;
+    if (L > cap) {
+      V += (L + cap - 1) / cap - 1;
+      ++P.nsplit;
     }
-    ++rows_in;
-    nnz_in += L;
   }
-  starts.push_back(A.rows);
+  if (V >= (long long)INT32_MAX) return hipErrorInvalidValue;
+  P.vrows = (int)V;
+  if (P.nsplit > 0) {
+    vrow_h.resize((size_t)A.rows + 1);
+    int v = 0;
+    for (int r = 0; r < A.rows; ++r) {
+      const long long L = off_h[r + 1] - off_h[r];
+      vrow_h[r] = v;
+      v += L > cap ? (int)((L + cap - 1) / cap) : 1;
+    }
+    vrow_h[A.rows] = v;
+  }
+  std::vector<long long> bnnz;
+  for (int tries = 0;; ++tries) {
+    starts.assign(1, 0);
+    bnnz.clear();
+    int rows_in = 0, v = 0;
+    long long nnz_in = 0;
+    auto add = [&](long long L) {
+      if (rows_in > 0 && (rows_in == kBlkRows ||
+                          (nnz_in + L > target && nnz_in + L - target > target - nnz_in))) {
+        starts.push_back(v);
+        bnnz.push_back(nnz_in);
+        rows_in = 0;
+        nnz_in = 0;
+      }
+      ++rows_in;
+      nnz_in += L;
+      ++v;
+    };
+    for (int r = 0; r < A.rows; ++r) {
+      const long long L = off_h[r + 1] - off_h[r];
+      const long long m = L > cap ? (L + cap - 1) / cap : 1;
+      for (long long i = 0; i < m; ++i) add((L - i + m - 1) / m);
+    }
+    starts.push_back(v);
+    bnnz.push_back(nnz_in);
+    if ((long long)bnnz.size() <= want || tries == 63) break;
+    target += target / 64 + 1;
+  }
   P.nb = (int)starts.size() - 1;
   // Launch order: heaviest blocks first.
   std::vector<int> order((size_t)P.nb);
   for (int b = 0; b < P.nb; ++b) order[b] = b;
-  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
-    return off_h[starts[u + 1]] - off_h[starts[u]] > off_h[starts[v + 1]] - off_h[starts[v]];
-  });
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int u, int w) { return bnnz[u] > bnnz[w]; });
   const int nch = (P.W + kBlkChunkCols - 1) / kBlkChunkCols;
   const long long nstreams = (long long)P.nb * P.S;
   const long long nbuckets = nstreams * nch;
   if (nbuckets + 1 >= (long long)INT32_MAX) return hipErrorInvalidValue;
   const int vbytes = A.vtype == LOOPS_VALUE_F32 ? 4 : 2;
-  P.part_stride = (long long)((A.rows + 63) & ~63);
+  P.part_stride = (V + 63) & ~63LL;
 
   // Build scratch: bucket counts, bucket starts (+1), rounds per stream and
   // their exclusive prefix (+1), scan scratch.
@@ -485,8 +561,9 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const size_t b_trow = al256(((size_t)A.nnz + 8) * sizeof(uint16_t));
   const size_t b_bstart = al256(((size_t)P.nb + 1) * sizeof(int));
   const size_t b_order = al256(((size_t)P.nb + 1) * sizeof(int));
+  const size_t b_vrow = al256(vrow_h.size() * sizeof(int));
   char* sc = nullptr;
-  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart);
+  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow);
   if (e != hipSuccess) return e;
   int* cnt = reinterpret_cast<int*>(sc);
   int* start = reinterpret_cast<int*>(sc + b_cnt);
@@ -495,7 +572,10 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   void* scan_scratch = sc + b_cnt + b_start + 2 * b_nr;
   uint16_t* trow = reinterpret_cast<uint16_t*>(sc + b_cnt + b_start + 2 * b_nr + b_scan);
   int* bstart = reinterpret_cast<int*>(sc + b_cnt + b_start + 2 * b_nr + b_scan + b_trow);
-  const blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart};
+  int* vrow = P.nsplit > 0 ? reinterpret_cast<int*>(sc + b_cnt + b_start + 2 * b_nr + b_scan +
+                                                    b_trow + b_bstart)
+                           : nullptr;
+  const blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart, vrow};
   long long grid = ((long long)A.nnz + 255) / 256;
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
@@ -506,6 +586,9 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   int nrounds = 0;
   e = hipMemcpyAsync(bstart, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice,
                      s);
+  if (e == hipSuccess && vrow)
+    e = hipMemcpyAsync(vrow, vrow_h.data(), vrow_h.size() * sizeof(int), hipMemcpyHostToDevice,
+                       s);
   if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, b_cnt, s);
   if (e == hipSuccess && A.nnz > 0) {
     blk_count_kernel<<<(unsigned)grid, 256, 0, s>>>(A.offsets, A.indices, g, cnt);
@@ -536,7 +619,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const size_t b_roff = al256((size_t)(nrounds + 1) * sizeof(int));
   const size_t b_part = al256((size_t)P.S * P.part_stride * sizeof(float));
   const size_t total =
-      b_packed + b_vals + b_srow + b_sround + b_roff + b_part + b_bstart + b_order;
+      b_packed + b_vals + b_srow + b_sround + b_roff + b_part + b_bstart + b_order + b_vrow;
   if ((e = hipMalloc(&P.mem, total)) != hipSuccess) {
     (void)hipFree(sc);
     return e;
@@ -551,6 +634,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   P.blk_start = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow + b_sround + b_roff +
                                        b_part);
   P.blk_order = P.blk_start + b_bstart / sizeof(int);
+  if (vrow) P.vrow = P.blk_order + b_order / sizeof(int);
   P.device_bytes = total;
 
   e = hipMemsetAsync(P.mem, 0, b_packed + b_vals + b_srow, s);
@@ -563,6 +647,8 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   if (e == hipSuccess)
     e = hipMemcpyAsync(P.blk_order, order.data(), (size_t)P.nb * sizeof(int),
                        hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && vrow)
+    e = hipMemcpyAsync(P.vrow, vrow, vrow_h.size() * sizeof(int), hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) {
     blk_round_write_kernel<<<sgrid, 256, 0, s>>>(start, (int)nstreams, nch, P.stream_round,
                                                  A.nnz, P.round_off);
@@ -606,7 +692,7 @@ hipError_t blk_plan_execute(const blk_plan_state& st, const float* x, float* y,
   long long grid = ((long long)st.rows + 255) / 256;
   if (grid > 8192) grid = 8192;
   spmv_blocks_combine_kernel<<<(unsigned)grid, 256, 0, s>>>(st.partial, st.part_stride, st.S,
-                                                             st.rows, y);
+                                                             st.rows, st.vrow, y);
   return hipGetLastError();
 }
 

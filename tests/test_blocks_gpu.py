@@ -3,7 +3,8 @@ blocks of 15872 rows x column segments (4, or a multiple of 8 for more than
 2^22 columns), each (block, segment) stream bucketed by 64-column chunks
 and cut into rounds of at most 4096 entries sorted by (row, column); y
 accumulated in LDS one round at a time, the segments' partial rows added in
-segment order.
+segment order; one-entry rows written directly by the workgroup of their
+column segment (y starts as NaN, so a row nobody writes fails).
 
 Bar: per row |y - y64| <= 1e-5 * sum_j |a_ij x_j| against the FP64 oracle and
 the reference validator's count of |y - y32| > 1e-2 equal to 0
@@ -46,6 +47,23 @@ def _mats():
     # 64-column chunks with far more than 4096 entries per block: split rounds
     off, idx, val = random_csr(20000, 40, 40, seed=13, empty_frac=0.05)
     m["dense_narrow"] = (off, idx, val, 40)
+    # rows of 0 or 1 entries only: blocks without kept rows, every value
+    # written by the one-entry pass
+    rng = np.random.default_rng(17)
+    lens = rng.integers(0, 2, 30000)
+    off = np.zeros(lens.shape[0] + 1, np.int32)
+    np.cumsum(lens, out=off[1:])
+    idx = rng.integers(0, 700000, int(lens.sum()))
+    m["one_entry_rows"] = (off, idx.astype(np.int32),
+                           rng.uniform(-1, 1, idx.shape[0]).astype(np.float32), 700000)
+    # one-entry, empty and kept rows interleaved, 4 and 8 column segments
+    for cols, name in ((700000, "mixed_lengths_4seg"), (5_000_000, "mixed_lengths_8seg")):
+        lens = np.tile(np.array([1, 2, 0, 1, 1, 5, 1, 17], np.int64), 4000)
+        off = np.zeros(lens.shape[0] + 1, np.int32)
+        np.cumsum(lens, out=off[1:])
+        idx = np.concatenate([np.sort(rng.choice(cols, L, replace=False)) for L in lens])
+        m[name] = (off, idx.astype(np.int32),
+                   rng.uniform(-1, 1, idx.shape[0]).astype(np.float32), cols)
     return m
 
 

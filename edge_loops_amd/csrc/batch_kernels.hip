@@ -491,9 +491,11 @@ loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
   } else {
     if (b.schedule != LOOPS_SCHEDULE_MERGE_PATH_FLAT) return LOOPS_ERROR_NOT_SUPPORTED;
     c->kind = BK_SPMM;
-    c->ipt = b.items_per_thread ? b.items_per_thread : 8;
-    if (c->ipt != 4 && c->ipt != 8 && c->ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
     const spmm_pick sp = spmm_shape_for(b);
+    // 0: the per-call default (16 steps for ranges of 16 lanes or more, else
+    // 8), so an item matches loops_spmm with the same parameters bit for bit
+    c->ipt = b.items_per_thread ? b.items_per_thread : (sp.g >= 16 ? 16 : 8);
+    if (c->ipt != 4 && c->ipt != 8 && c->ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
     c->g = sp.g;
     c->cpl = sp.cpl;
     c->ti = (int)spmm_tile_items(c->g, c->ipt);

@@ -227,9 +227,9 @@ loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
  *   k == 1, LOOPS_SCHEDULE_GROUP_MAPPED: items_per_thread lanes per row
  *           (2, 4, ..., 64; 0 = the power of two at or above the mean row
  *           length), reduced in a fixed order;
- *   k >  1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge-path units of
- *           items_per_thread steps (16..65536; 0 = 256) walked by lane
- *           groups spanning the k columns.
+ *   k >  1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge-path units of lane
+ *           groups spanning the k columns, items_per_thread steps per group
+ *           (4, 8 or 16; 0 = the loops_spmm default).
  * Values fp32, bf16 or fp16; indices / values 16-byte aligned (8 for 16-bit
  * values).
  * Steps: loops_batch_query (sizes) -> loops_batch_prepare (host only: fills
@@ -369,7 +369,10 @@ loops_status_t loops_server_replay(loops_server_t server,
  * (container/matrix.cuh:27-35) with leading dimensions ldb, ldc >= k.
  * Schedules: THREAD_MAPPED (algorithms/spmm/thread_mapped.cuh:28-53),
  * GROUP_MAPPED and MERGE_PATH_FLAT (new in this library; the reference has
- * only thread_mapped SpMM). */
+ * only thread_mapped SpMM).  MERGE_PATH_FLAT: params->items_per_thread merge
+ * steps per lane range (4, 8 or 16; 0 = 16 when a range spans 16 lanes or
+ * more, i.e. k above 32 with 16-byte aligned B and C rows or k above 8
+ * otherwise, else 8). */
 loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
                                          loops_schedule_t schedule,
                                          const loops_csr_view_t* A, int64_t k,

@@ -118,52 +118,72 @@ __global__ __launch_bounds__(256) void blk_count_kernel(const int* __restrict__ 
 // Rounds of stream q: its chunks in order, packed greedily into rounds of at
 // most kBlkRound entries; a chunk larger than that is cut into full rounds
 // and a remainder that later chunks may join.  With off == nullptr only
-// counts them (nrounds[q]); otherwise writes each round's first entry at
-// off[first[q] ...].
+// counts them; otherwise writes each round's first entry at off[0 ...].
+// One wavefront per stream: the lanes load 64 chunk starts at a time
+// (coalesced) and the whole wavefront runs the packing on readlane values,
+// so the sequential walk waits on memory once per 64 chunks.
 __device__ __forceinline__ int blk_rounds_of(const int* __restrict__ start, long long q,
                                              int nch, int* __restrict__ off) {
+  const int lane = threadIdx.x & 63;
   const long long k0 = q * nch;
-  int n = 0, cur = 0, cur_start = start[k0];
-  for (int c = 0; c < nch; ++c) {
-    const int a = start[k0 + c], size = start[k0 + c + 1] - a;
-    if (size == 0) continue;
-    if (cur + size <= kBlkRound) {
-      if (cur == 0) cur_start = a;
-      cur += size;
-      continue;
+  int n = 0, cur = 0, cur_start = 0;
+  for (int c0 = 0; c0 < nch; c0 += 64) {
+    const int c = c0 + lane;
+    int a = 0, size = 0;
+    if (c < nch) {
+      a = start[k0 + c];
+      size = start[k0 + c + 1] - a;
     }
-    if (cur > 0) {
-      if (off) off[n] = cur_start;
-      ++n;
+    const int m = nch - c0 < 64 ? nch - c0 : 64;
+    for (int i = 0; i < m; ++i) {
+      const int ai = __builtin_amdgcn_readlane(a, i);
+      const int si = __builtin_amdgcn_readlane(size, i);
+      if (si == 0) continue;
+      if (cur + si <= kBlkRound) {
+        if (cur == 0) cur_start = ai;
+        cur += si;
+        continue;
+      }
+      if (cur > 0) {
+        if (off && lane == 0) off[n] = cur_start;
+        ++n;
+      }
+      int done = 0;
+      while (si - done > kBlkRound) {
+        if (off && lane == 0) off[n] = ai + done;
+        ++n;
+        done += kBlkRound;
+      }
+      cur = si - done;
+      cur_start = ai + done;
     }
-    int done = 0;
-    while (size - done > kBlkRound) {
-      if (off) off[n] = a + done;
-      ++n;
-      done += kBlkRound;
-    }
-    cur = size - done;
-    cur_start = a + done;
   }
   if (cur > 0) {
-    if (off) off[n] = cur_start;
+    if (off && lane == 0) off[n] = cur_start;
     ++n;
   }
   return n;
 }
 
-__global__ void blk_round_count_kernel(const int* __restrict__ start, int nstreams, int nch,
-                                       int* __restrict__ nrounds) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < nstreams) nrounds[q] = blk_rounds_of(start, q, nch, nullptr);
+// One wavefront per stream (blockDim.x a multiple of 64).
+__global__ __launch_bounds__(256) void blk_round_count_kernel(const int* __restrict__ start,
+                                                              int nstreams, int nch,
+                                                              int* __restrict__ nrounds) {
+  const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (q >= nstreams) return;
+  const int n = blk_rounds_of(start, q, nch, nullptr);
+  if ((threadIdx.x & 63) == 0) nrounds[q] = n;
 }
 
-__global__ void blk_round_write_kernel(const int* __restrict__ start, int nstreams, int nch,
-                                       const int* __restrict__ first, int nnz,
-                                       int* __restrict__ round_off) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < nstreams) blk_rounds_of(start, q, nch, round_off + first[q]);
-  if (q == nstreams) round_off[first[nstreams]] = nnz;
+__global__ __launch_bounds__(256) void blk_round_write_kernel(const int* __restrict__ start,
+                                                              int nstreams, int nch,
+                                                              const int* __restrict__ first,
+                                                              int nnz,
+                                                              int* __restrict__ round_off) {
+  const long long q = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (q == 0 && (threadIdx.x & 63) == 0) round_off[first[nstreams]] = nnz;
+  if (q >= nstreams) return;
+  blk_rounds_of(start, q, nch, round_off + first[q]);
 }
 
 // Every nonzero to the next free slot of its bucket (cursor = bucket
@@ -579,7 +599,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   long long grid = ((long long)A.nnz + 255) / 256;
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
-  const unsigned sgrid = (unsigned)((nstreams + 1 + 255) / 256);
+  const unsigned sgrid = (unsigned)((nstreams + 3) / 4 > 0 ? (nstreams + 3) / 4 : 1);  // wave per stream
 
   // Plan memory, part 1: stream -> first round (needed to count rounds).
   const size_t b_sround = al256((size_t)(nstreams + 1) * sizeof(int));

@@ -257,15 +257,16 @@ static_assert(kBlkIpt == 8 && kBlkIpt * kBlkTpb == kBlkRound,
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 // What a lane reads of one round: its phase-1 entries (wavefront w: entries
-// 512 w .. 512 w + 511, lane l: 64 j + l) and its phase-2 slots 8 t .. 8 t + 7
-// with the rows of the slots before and after them.
+// 512 w .. 512 w + 511, lane l: 64 j + l) and the rows of its phase-2 slots
+// 8 t .. 8 t + 7; the wave also reads the rows of the slots just before and
+// after its 512.
 template <typename V>
 struct blk_round_regs {
   int a, n;
   uint32_t pw[kBlkIpt];
   uint32_t vw[kBlkIpt];  // raw value words (blk_word_value)
   u16x8 rows;
-  int prev_row, next_row;
+  int edge_prev, edge_next;  // wave-uniform: the rows of the slots just before / after the wave's
 };
 
 // Values travel as raw 32-bit words and are converted where they are
@@ -311,13 +312,16 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
   }
   const uint16_t* rs = srow + (long long)r * kBlkRound;
   g.rows = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-  g.prev_row = -1;
-  g.next_row = -1;
-  if (k0 < g.n) {
-    g.rows = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(rs + k0));
-    if (k0 > 0) g.prev_row = rs[k0 - 1];
-    if (k0 + kBlkIpt < g.n) g.next_row = rs[k0 + kBlkIpt];
-  }
+  if (k0 < g.n) g.rows = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(rs + k0));
+  // The neighbours' rows inside the wave come from lane shuffles (phase 2);
+  // only the two slots at the wave's edges are loaded, as aligned words at
+  // wave-uniform addresses.
+  const uint32_t* rw = reinterpret_cast<const uint32_t*>(rs);
+  const int base = __builtin_amdgcn_readfirstlane(wave) * 64 * kBlkIpt;
+  g.edge_prev = -1;
+  g.edge_next = -1;
+  if (base > 0 && base < g.n) g.edge_prev = (int)(rw[(base - 2) >> 1] >> 16);
+  if (base + 64 * kBlkIpt < g.n) g.edge_next = (int)(rw[(base + 64 * kBlkIpt) >> 1] & 0xffffu);
 }
 
 template <typename V>
@@ -362,8 +366,14 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
         stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], cur.a + i) * xv[j];
     }
     __syncthreads();
-    // Phase 2: runs of equal rows over slots k0 .. k0 + m - 1.
+    // Phase 2: runs of equal rows over slots k0 .. k0 + m - 1.  prev_row /
+    // next_row: the rows of slots k0 - 1 and k0 + kBlkIpt (-1 outside the
+    // round), from the neighbouring lanes or the wave's edge slots.
     const int m = n - k0 < 0 ? 0 : (n - k0 > kBlkIpt ? kBlkIpt : n - k0);
+    const int up_row = __shfl_up((int)cur.rows[kBlkIpt - 1], 1, 64);
+    const int down_row = __shfl_down((int)cur.rows[0], 1, 64);
+    const int prev_row = k0 == 0 ? -1 : (lane == 0 ? cur.edge_prev : up_row);
+    const int next_row = k0 + kBlkIpt >= n ? -1 : (lane == 63 ? cur.edge_next : down_row);
     float v = 0.f, first_sum = 0.f;
     int f = 1, first_row = -1, row = cur.rows[0];
     bool internal = false;
@@ -389,7 +399,7 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
           }
         }
       }
-      f = internal || cur.prev_row != (int)cur.rows[0];
+      f = internal || prev_row != (int)cur.rows[0];
     }
     const float v_own = v;
     float pv;
@@ -399,10 +409,10 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     const float up = __shfl_up(v, 1, 64);
     const float ex = lane == 0 ? pv : up;
     if (m > 0) {
-      const bool cont = k0 > 0 && cur.prev_row == (int)cur.rows[0];
+      const bool cont = k0 > 0 && prev_row == (int)cur.rows[0];
       const float cin = cont ? ex : 0.f;
       if (internal) acc[first_row] += cin + first_sum;
-      if (k0 + m >= n || cur.next_row != row) acc[row] += (internal ? 0.f : cin) + v_own;
+      if (k0 + m >= n || next_row != row) acc[row] += (internal ? 0.f : cin) + v_own;
     }
     if (r + 1 < re) cur = nxt;
   }

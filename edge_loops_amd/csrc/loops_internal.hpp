@@ -103,8 +103,10 @@ constexpr int kBlkChunkCols = 64;               // columns per build bucket
 struct blk_plan_state {
   int rows = 0, cols = 0, nnz = 0, S = 4, W = 1, nb = 0, nrounds = 0;
   loops_value_type_t vtype = LOOPS_VALUE_F32;
-  uint32_t* packed = nullptr;   // column in segment << kBlkSlotBits | slot in round
-  void* values = nullptr;       // column order within the round
+  // column in segment << kBlkSlotBits | slot in round, in column order within
+  // the round; fp32 plans pair each with its value's bits (8-byte records)
+  uint32_t* packed = nullptr;
+  void* values = nullptr;       // 16-bit values (null for fp32 plans)
   uint16_t* srow = nullptr;     // row in block of slot k of round r at r * kBlkRound + k
   int* stream_round = nullptr;  // first round of each (block, segment) stream
   int* round_off = nullptr;     // first entry of each round (nrounds + 1)

@@ -36,7 +36,9 @@
 // partials of each row in segment order (and a cut row's pieces in piece
 // order).
 //
-// Bytes per fp32 entry: 4 (value) + 4 (column | slot) + 2 (row of a slot).
+// Bytes per fp32 entry: 8 (column | slot and value, one record, so a lane
+// loads both with one instruction) + 2 (row of a slot); 16-bit values sit in
+// an array of their own beside the column | slot words.
 // The loads of round r + 1 are issued while round r's x reads are in
 // flight, so each round waits for memory once.
 //
@@ -246,6 +248,15 @@ __global__ __launch_bounds__(kBlkSortTpb) void blk_round_sort_kernel(
   }
 }
 
+// fp32 plans: entry e as one record (packed word, value bits).
+__global__ __launch_bounds__(256) void blk_pair_kernel(const uint32_t* __restrict__ pk,
+                                                       const uint32_t* __restrict__ vv, int nnz,
+                                                       u32x2* __restrict__ out) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+       e += (long long)gridDim.x * blockDim.x)
+    out[e] = u32x2{pk[e], vv[e]};
+}
+
 // ---------------------------------------------------------------------------
 // Execute kernels
 // ---------------------------------------------------------------------------
@@ -306,8 +317,15 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
     g.pw[j] = 0;
     g.vw[j] = 0;
     if (i < g.n) {
-      g.pw[j] = (uint32_t)ld_idx(reinterpret_cast<const int*>(packed + g.a + i));
-      g.vw[j] = blk_load_word(vals, g.a + i);
+      if constexpr (std::is_same<V, float>::value) {
+        const u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(packed) +
+                                                   (g.a + i));
+        g.pw[j] = w.x;
+        g.vw[j] = w.y;
+      } else {
+        g.pw[j] = (uint32_t)ld_idx(reinterpret_cast<const int*>(packed + g.a + i));
+        g.vw[j] = blk_load_word(vals, g.a + i);
+      }
     }
   }
   const uint16_t* rs = srow + (long long)r * kBlkRound;
@@ -460,10 +478,10 @@ inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
 template <typename V>
 hipError_t blk_scatter(const csr_args& A, const blk_geom& g, int* cursor, uint16_t* trow,
-                       blk_plan_state& P, unsigned grid, hipStream_t s) {
+                       uint32_t* pk, void* vv, unsigned grid, hipStream_t s) {
   blk_scatter_kernel<V><<<grid, 256, 0, s>>>(A.offsets, A.indices,
-                                             static_cast<const V*>(A.values), g, cursor,
-                                             P.packed, trow, static_cast<V*>(P.values));
+                                             static_cast<const V*>(A.values), g, cursor, pk,
+                                             trow, static_cast<V*>(vv));
   return hipGetLastError();
 }
 
@@ -592,8 +610,14 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const size_t b_bstart = al256(((size_t)P.nb + 1) * sizeof(int));
   const size_t b_order = al256(((size_t)P.nb + 1) * sizeof(int));
   const size_t b_vrow = al256(vrow_h.size() * sizeof(int));
+  // fp32 plans store each entry as one 8-byte record (column | slot, value);
+  // the build works on separate arrays here and interleaves them at the end.
+  const bool pairs = A.vtype == LOOPS_VALUE_F32;
+  const size_t n_pad = (size_t)A.nnz + 8;
+  const size_t b_work = pairs ? al256(n_pad * sizeof(uint32_t)) : 0;
   char* sc = nullptr;
-  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow);
+  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow +
+                         2 * b_work);
   if (e != hipSuccess) return e;
   int* cnt = reinterpret_cast<int*>(sc);
   int* start = reinterpret_cast<int*>(sc + b_cnt);
@@ -605,6 +629,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   int* vrow = P.nsplit > 0 ? reinterpret_cast<int*>(sc + b_cnt + b_start + 2 * b_nr + b_scan +
                                                     b_trow + b_bstart)
                            : nullptr;
+  char* work = sc + b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow;
   const blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart, vrow};
   long long grid = ((long long)A.nnz + 255) / 256;
   if (grid > 65536) grid = 65536;
@@ -642,9 +667,8 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
 
   // Plan memory, part 2: entries, rows per slot (kBlkRound per round),
   // rounds, partials.
-  const size_t n_pad = (size_t)A.nnz + 8;
-  const size_t b_packed = al256(n_pad * sizeof(uint32_t));
-  const size_t b_vals = al256(n_pad * vbytes);
+  const size_t b_packed = al256(n_pad * (pairs ? 2 * sizeof(uint32_t) : sizeof(uint32_t)));
+  const size_t b_vals = pairs ? 0 : al256(n_pad * vbytes);
   const size_t b_srow = al256(((size_t)nrounds * kBlkRound + 8) * sizeof(uint16_t));
   const size_t b_roff = al256((size_t)(nrounds + 1) * sizeof(int));
   const size_t b_part = al256((size_t)P.S * P.part_stride * sizeof(float));
@@ -656,7 +680,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   }
   char* m = static_cast<char*>(P.mem);
   P.packed = reinterpret_cast<uint32_t*>(m);
-  P.values = m + b_packed;
+  P.values = pairs ? nullptr : m + b_packed;
   P.srow = reinterpret_cast<uint16_t*>(m + b_packed + b_vals);
   P.stream_round = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow);
   P.round_off = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow + b_sround);
@@ -685,22 +709,29 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
     e = hipGetLastError();
   }
   // the bucket starts become the scatter cursors
+  uint32_t* pk = pairs ? reinterpret_cast<uint32_t*>(work) : P.packed;
+  void* vv = pairs ? work + b_work : P.values;
   if (e == hipSuccess && A.nnz > 0) {
     switch (A.vtype) {
       case LOOPS_VALUE_F32:
-        e = blk_scatter<float>(A, g, start, trow, P, (unsigned)grid, s);
+        e = blk_scatter<float>(A, g, start, trow, pk, vv, (unsigned)grid, s);
         break;
       case LOOPS_VALUE_BF16:
-        e = blk_scatter<bf16_t>(A, g, start, trow, P, (unsigned)grid, s);
+        e = blk_scatter<bf16_t>(A, g, start, trow, pk, vv, (unsigned)grid, s);
         break;
       case LOOPS_VALUE_F16:
-        e = blk_scatter<f16_t>(A, g, start, trow, P, (unsigned)grid, s);
+        e = blk_scatter<f16_t>(A, g, start, trow, pk, vv, (unsigned)grid, s);
         break;
     }
   }
   if (e == hipSuccess && nrounds > 0) {
-    blk_round_sort_kernel<<<(unsigned)nrounds, kBlkSortTpb, 0, s>>>(P.round_off, P.packed,
-                                                                     trow, P.srow);
+    blk_round_sort_kernel<<<(unsigned)nrounds, kBlkSortTpb, 0, s>>>(P.round_off, pk, trow,
+                                                                     P.srow);
+    e = hipGetLastError();
+  }
+  if (pairs && e == hipSuccess && A.nnz > 0) {
+    blk_pair_kernel<<<(unsigned)grid, 256, 0, s>>>(pk, static_cast<const uint32_t*>(vv), A.nnz,
+                                                   reinterpret_cast<u32x2*>(P.packed));
     e = hipGetLastError();
   }
   const hipError_t e2 = hipStreamSynchronize(s);

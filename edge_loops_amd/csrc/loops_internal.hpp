@@ -103,13 +103,16 @@ constexpr int kBlkChunkCols = 64;               // columns per build bucket
 struct blk_plan_state {
   int rows = 0, cols = 0, nnz = 0, S = 4, W = 1, nb = 0, nrounds = 0;
   loops_value_type_t vtype = LOOPS_VALUE_F32;
-  // column in segment << kBlkSlotBits | slot in round, in column order within
-  // the round; fp32 plans pair each with its value's bits (8-byte records)
+  // Entries of a round in column order, stored lane-major per 512-entry slab
+  // (entry 512 q + 64 j + l at 512 q + 256 (j / 4) + 4 l + j % 4; 16-bit
+  // values at 512 q + 8 l + j), each round from slab round_base[r] on:
+  // column in segment << kBlkSlotBits | slot in round, and the values.
   uint32_t* packed = nullptr;
-  void* values = nullptr;       // 16-bit values (null for fp32 plans)
+  void* values = nullptr;
   uint16_t* srow = nullptr;     // row in block of slot k of round r at r * kBlkRound + k
   int* stream_round = nullptr;  // first round of each (block, segment) stream
-  int* round_off = nullptr;     // first entry of each round (nrounds + 1)
+  int* round_off = nullptr;     // first entry of each round (nrounds + 1), stream order
+  int* round_base = nullptr;    // first slab of each round in packed / values (nrounds + 1)
   int* blk_start = nullptr;     // first row of each block (nb + 1)
   int* blk_order = nullptr;     // blocks in launch order (heaviest first)
   float* partial = nullptr;     // S arrays of part_stride floats

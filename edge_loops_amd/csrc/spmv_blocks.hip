@@ -36,9 +36,12 @@
 // partials of each row in segment order (and a cut row's pieces in piece
 // order).
 //
-// Bytes per fp32 entry: 8 (column | slot and value, one record, so a lane
-// loads both with one instruction) + 2 (row of a slot); 16-bit values sit in
-// an array of their own beside the column | slot words.
+// Bytes per fp32 entry: 4 (value) + 4 (column | slot) + 2 (row of a slot).
+// Entries are stored lane-major in slabs of 512 (a wavefront's share of a
+// round), so a lane loads its 8 column | slot words with two 16-byte loads
+// and its values with two (fp32) or one (16-bit), each load a contiguous
+// kilobyte across the wavefront; rounds start at whole slabs, which pads
+// each round by less than 512 entries.
 // The loads of round r + 1 are issued while round r's x reads are in
 // flight, so each round waits for memory once.
 //
@@ -59,6 +62,12 @@
 #include "loops_internal.hpp"
 
 namespace loops_mi355x {
+
+constexpr int kBlkTpb = 512;
+constexpr int kBlkIpt = kBlkRound / kBlkTpb;  // entries per lane per phase (8)
+constexpr int kBlkSlab = 64 * kBlkIpt;        // entries of one wavefront per round
+static_assert(kBlkIpt == 8 && kBlkIpt * kBlkTpb == kBlkRound,
+              "a round is one pass of the workgroup, 8 slots (16 bytes of rows) per lane");
 
 // ---------------------------------------------------------------------------
 // Build kernels
@@ -248,56 +257,59 @@ __global__ __launch_bounds__(kBlkSortTpb) void blk_round_sort_kernel(
   }
 }
 
-// fp32 plans: entry e as one record (packed word, value bits).
-__global__ __launch_bounds__(256) void blk_pair_kernel(const uint32_t* __restrict__ pk,
-                                                       const uint32_t* __restrict__ vv, int nnz,
-                                                       u32x2* __restrict__ out) {
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
-       e += (long long)gridDim.x * blockDim.x)
-    out[e] = u32x2{pk[e], vv[e]};
+// Stores each round lane-major: within slab q (entries 512 q .. 512 q + 511
+// of the round, column order; lane l of wavefront q takes entries
+// 512 q + 64 j + l, j = 0..7) a lane's 4-byte words j = 0..3 sit at
+// 512 q + 4 l + j and j = 4..7 at 512 q + 256 + 4 l + j - 4, so each of its
+// two 16-byte loads is one contiguous kilobyte across the wavefront; 16-bit
+// values sit at 512 q + 8 l + j (one 16-byte load).  Rounds start at whole
+// slabs (base[r] slabs); positions past a round's end stay 0.  One workgroup
+// per round.
+template <typename V>
+__global__ __launch_bounds__(256) void blk_lane_order_kernel(
+    const int* __restrict__ round_off, const int* __restrict__ base,
+    const uint32_t* __restrict__ pk, const V* __restrict__ vv, uint32_t* __restrict__ packed,
+    V* __restrict__ vals) {
+  const int r = blockIdx.x;
+  const int a = round_off[r], n = round_off[r + 1] - a;
+  const long long b = (long long)base[r] * kBlkSlab;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int q = i / kBlkSlab, t = i % kBlkSlab, j = t / 64, l = t % 64;
+    const long long s0 = b + (long long)q * kBlkSlab;
+    const int w = (j / 4) * (kBlkSlab / 2) + l * 4 + j % 4;
+    packed[s0 + w] = pk[a + i];
+    vals[s0 + (std::is_same<V, float>::value ? w : l * kBlkIpt + j)] = vv[a + i];
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Execute kernels
 // ---------------------------------------------------------------------------
-constexpr int kBlkTpb = 512;
-constexpr int kBlkIpt = kBlkRound / kBlkTpb;  // entries per lane per phase (8)
-static_assert(kBlkIpt == 8 && kBlkIpt * kBlkTpb == kBlkRound,
-              "a round is one pass of the workgroup, 8 slots (16 bytes of rows) per lane");
 
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 // What a lane reads of one round: its phase-1 entries (wavefront w: entries
-// 512 w .. 512 w + 511, lane l: 64 j + l) and the rows of its phase-2 slots
-// 8 t .. 8 t + 7; the wave also reads the rows of the slots just before and
-// after its 512.
+// 512 w .. 512 w + 511, lane l: 64 j + l, stored lane-major so that the
+// lane's 8 are consecutive) and the rows of its phase-2 slots 8 t .. 8 t + 7;
+// the wave also reads the rows of the slots just before and after its 512.
 template <typename V>
 struct blk_round_regs {
-  int a, n;
+  int n;
   uint32_t pw[kBlkIpt];
   uint32_t vw[kBlkIpt];  // raw value words (blk_word_value)
   u16x8 rows;
   int edge_prev, edge_next;  // wave-uniform: the rows of the slots just before / after the wave's
 };
 
-// Values travel as raw 32-bit words and are converted where they are
-// used, so no load result is waited for inside the guarded load blocks.
-// 16-bit values are read as the aligned 32-bit word holding the pair (the
-// value arrays are padded, so the word stays in bounds); entry e takes the
-// half e & 1.
-__device__ __forceinline__ uint32_t blk_load_word(const float* vals, int e) {
-  return (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const int*>(vals + e));
-}
+// Values travel as raw 32-bit words and are converted where they are used,
+// so no load result is waited for inside the guarded load block.  A lane's
+// 8 16-bit values are 4 words; value j is half j & 1 of word j / 2.
 template <typename V>
-__device__ __forceinline__ uint32_t blk_load_word(const V* vals, int e) {
-  return (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const int*>(vals + (e & ~1)));
-}
-template <typename V>
-__device__ __forceinline__ float blk_word_value(uint32_t w, int e) {
+__device__ __forceinline__ float blk_word_value(uint32_t w, int j) {
   if constexpr (std::is_same<V, float>::value) {
     return __uint_as_float(w);
   } else {
-    const uint32_t h = (e & 1) ? (w >> 16) : (w & 0xffffu);
+    const uint32_t h = (j & 1) ? (w >> 16) : (w & 0xffffu);
     if constexpr (std::is_same<V, bf16_t>::value) return __uint_as_float(h << 16);
     else return static_cast<float>(__builtin_bit_cast(f16_t, (uint16_t)h));
   }
@@ -307,25 +319,34 @@ template <typename V>
 __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict__ packed,
                                                const V* __restrict__ vals,
                                                const uint16_t* __restrict__ srow,
-                                               const int* __restrict__ round_off, int lane,
+                                               const int* __restrict__ round_off,
+                                               const int* __restrict__ round_base, int lane,
                                                int wave, int k0, blk_round_regs<V>& g) {
-  g.a = round_off[r];
-  g.n = round_off[r + 1] - g.a;
+  g.n = round_off[r + 1] - round_off[r];
 #pragma unroll
   for (int j = 0; j < kBlkIpt; ++j) {
-    const int i = wave * 64 * kBlkIpt + j * 64 + lane;
     g.pw[j] = 0;
     g.vw[j] = 0;
-    if (i < g.n) {
-      if constexpr (std::is_same<V, float>::value) {
-        const u32x2 w = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(packed) +
-                                                   (g.a + i));
-        g.pw[j] = w.x;
-        g.vw[j] = w.y;
-      } else {
-        g.pw[j] = (uint32_t)ld_idx(reinterpret_cast<const int*>(packed + g.a + i));
-        g.vw[j] = blk_load_word(vals, g.a + i);
-      }
+  }
+  // The wave's slab exists when its first entry does; positions past the
+  // round's end inside it are zero padding.
+  if (wave * kBlkSlab < g.n) {
+    const long long s0 = ((long long)round_base[r] + wave) * kBlkSlab;
+    const long long p = s0 + lane * 4, p2 = p + kBlkSlab / 2;
+    const i32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p));
+    const i32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p2));
+    g.pw[0] = c0.x; g.pw[1] = c0.y; g.pw[2] = c0.z; g.pw[3] = c0.w;
+    g.pw[4] = c1.x; g.pw[5] = c1.y; g.pw[6] = c1.z; g.pw[7] = c1.w;
+    if constexpr (std::is_same<V, float>::value) {
+      const i32x4 v0 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(vals + p));
+      const i32x4 v1 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(vals + p2));
+      g.vw[0] = v0.x; g.vw[1] = v0.y; g.vw[2] = v0.z; g.vw[3] = v0.w;
+      g.vw[4] = v1.x; g.vw[5] = v1.y; g.vw[6] = v1.z; g.vw[7] = v1.w;
+    } else {
+      const i32x4 v = __builtin_nontemporal_load(
+          reinterpret_cast<const i32x4*>(vals + s0 + (long long)lane * kBlkIpt));
+      g.vw[0] = g.vw[1] = v.x; g.vw[2] = g.vw[3] = v.y;
+      g.vw[4] = g.vw[5] = v.z; g.vw[6] = g.vw[7] = v.w;
     }
   }
   const uint16_t* rs = srow + (long long)r * kBlkRound;
@@ -335,19 +356,20 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
   // only the two slots at the wave's edges are loaded, as aligned words at
   // wave-uniform addresses.
   const uint32_t* rw = reinterpret_cast<const uint32_t*>(rs);
-  const int base = __builtin_amdgcn_readfirstlane(wave) * 64 * kBlkIpt;
+  const int base = __builtin_amdgcn_readfirstlane(wave) * kBlkSlab;
   g.edge_prev = -1;
   g.edge_next = -1;
   if (base > 0 && base < g.n) g.edge_prev = (int)(rw[(base - 2) >> 1] >> 16);
-  if (base + 64 * kBlkIpt < g.n) g.edge_next = (int)(rw[(base + 64 * kBlkIpt) >> 1] & 0xffffu);
+  if (base + kBlkSlab < g.n) g.edge_next = (int)(rw[(base + kBlkSlab) >> 1] & 0xffffu);
 }
 
 template <typename V>
 __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     const uint32_t* __restrict__ packed, const V* __restrict__ vals,
     const uint16_t* __restrict__ srow, const int* __restrict__ stream_round,
-    const int* __restrict__ round_off, const int* __restrict__ blk_start,
-    const int* __restrict__ blk_order, const float* __restrict__ x, int S, int W,
+    const int* __restrict__ round_off, const int* __restrict__ round_base,
+    const int* __restrict__ blk_start, const int* __restrict__ blk_order,
+    const float* __restrict__ x, int S, int W,
     float* __restrict__ partial, long long part_stride) {
   __shared__ float acc[kBlkRows];
   __shared__ float stage[kBlkRound];
@@ -364,7 +386,8 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
   const int k0 = threadIdx.x * kBlkIpt;
   const int rb = stream_round[q], re = stream_round[q + 1];
   blk_round_regs<V> cur;
-  if (rb < re) blk_load_round<V>(rb, packed, vals, srow, round_off, lane, wave, k0, cur);
+  if (rb < re)
+    blk_load_round<V>(rb, packed, vals, srow, round_off, round_base, lane, wave, k0, cur);
   for (int r = rb; r < re; ++r) {
     const int n = cur.n;
     // Phase 1: products in column order to their slots; the next round's
@@ -376,12 +399,13 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
       xv[j] = i < n ? xs[cur.pw[j] >> kBlkSlotBits] : 0.f;
     }
     blk_round_regs<V> nxt;
-    if (r + 1 < re) blk_load_round<V>(r + 1, packed, vals, srow, round_off, lane, wave, k0, nxt);
+    if (r + 1 < re)
+      blk_load_round<V>(r + 1, packed, vals, srow, round_off, round_base, lane, wave, k0, nxt);
 #pragma unroll
     for (int j = 0; j < kBlkIpt; ++j) {
       const int i = wave * 64 * kBlkIpt + j * 64 + lane;
       if (i < n)
-        stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], cur.a + i) * xv[j];
+        stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], j) * xv[j];
     }
     __syncthreads();
     // Phase 2: runs of equal rows over slots k0 .. k0 + m - 1.  prev_row /
@@ -489,7 +513,7 @@ template <typename V>
 hipError_t blk_execute(const blk_plan_state& st, const float* x, hipStream_t s) {
   spmv_blocks_kernel<V><<<(unsigned)((long long)st.nb * st.S), kBlkTpb, 0, s>>>(
       st.packed, static_cast<const V*>(st.values), st.srow, st.stream_round, st.round_off,
-      st.blk_start, st.blk_order, x, st.S, st.W, st.partial, st.part_stride);
+      st.round_base, st.blk_start, st.blk_order, x, st.S, st.W, st.partial, st.part_stride);
   return hipGetLastError();
 }
 }  // namespace
@@ -610,14 +634,14 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const size_t b_bstart = al256(((size_t)P.nb + 1) * sizeof(int));
   const size_t b_order = al256(((size_t)P.nb + 1) * sizeof(int));
   const size_t b_vrow = al256(vrow_h.size() * sizeof(int));
-  // fp32 plans store each entry as one 8-byte record (column | slot, value);
-  // the build works on separate arrays here and interleaves them at the end.
-  const bool pairs = A.vtype == LOOPS_VALUE_F32;
+  // Entries in stream order (column | slot words, values) until they are
+  // stored lane-major in the plan.
   const size_t n_pad = (size_t)A.nnz + 8;
-  const size_t b_work = pairs ? al256(n_pad * sizeof(uint32_t)) : 0;
+  const size_t b_work = al256(n_pad * sizeof(uint32_t));
+  const size_t b_workv = al256(n_pad * vbytes);
   char* sc = nullptr;
-  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow +
-                         2 * b_work);
+  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow + b_work +
+                         b_workv);
   if (e != hipSuccess) return e;
   int* cnt = reinterpret_cast<int*>(sc);
   int* start = reinterpret_cast<int*>(sc + b_cnt);
@@ -665,30 +689,40 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   }
   P.nrounds = nrounds;
 
-  // Plan memory, part 2: entries, rows per slot (kBlkRound per round),
-  // rounds, partials.
-  const size_t b_packed = al256(n_pad * (pairs ? 2 * sizeof(uint32_t) : sizeof(uint32_t)));
-  const size_t b_vals = pairs ? 0 : al256(n_pad * vbytes);
+  // Plan memory, part 2: entries lane-major in whole slabs per round (at
+  // most 511 positions of padding per round), rows per slot (kBlkRound per
+  // round), rounds and their first slabs, partials, blocks.
+  const size_t n_lane = (size_t)A.nnz + (size_t)nrounds * (kBlkSlab - 1) + 8;
+  const size_t b_packed = al256(n_lane * sizeof(uint32_t));
+  const size_t b_vals = al256(n_lane * vbytes);
   const size_t b_srow = al256(((size_t)nrounds * kBlkRound + 8) * sizeof(uint16_t));
   const size_t b_roff = al256((size_t)(nrounds + 1) * sizeof(int));
   const size_t b_part = al256((size_t)P.S * P.part_stride * sizeof(float));
-  const size_t total =
-      b_packed + b_vals + b_srow + b_sround + b_roff + b_part + b_bstart + b_order + b_vrow;
+  size_t total = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = total;
+    total += bytes;
+    return at;
+  };
+  const size_t o_packed = take(b_packed), o_vals = take(b_vals), o_srow = take(b_srow);
+  const size_t o_sround = take(b_sround), o_roff = take(b_roff), o_rbase = take(b_roff);
+  const size_t o_part = take(b_part), o_bstart = take(b_bstart), o_order = take(b_order);
+  const size_t o_vrow = take(b_vrow);
   if ((e = hipMalloc(&P.mem, total)) != hipSuccess) {
     (void)hipFree(sc);
     return e;
   }
   char* m = static_cast<char*>(P.mem);
-  P.packed = reinterpret_cast<uint32_t*>(m);
-  P.values = pairs ? nullptr : m + b_packed;
-  P.srow = reinterpret_cast<uint16_t*>(m + b_packed + b_vals);
-  P.stream_round = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow);
-  P.round_off = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow + b_sround);
-  P.partial = reinterpret_cast<float*>(m + b_packed + b_vals + b_srow + b_sround + b_roff);
-  P.blk_start = reinterpret_cast<int*>(m + b_packed + b_vals + b_srow + b_sround + b_roff +
-                                       b_part);
-  P.blk_order = P.blk_start + b_bstart / sizeof(int);
-  if (vrow) P.vrow = P.blk_order + b_order / sizeof(int);
+  P.packed = reinterpret_cast<uint32_t*>(m + o_packed);
+  P.values = m + o_vals;
+  P.srow = reinterpret_cast<uint16_t*>(m + o_srow);
+  P.stream_round = reinterpret_cast<int*>(m + o_sround);
+  P.round_off = reinterpret_cast<int*>(m + o_roff);
+  P.round_base = reinterpret_cast<int*>(m + o_rbase);
+  P.partial = reinterpret_cast<float*>(m + o_part);
+  P.blk_start = reinterpret_cast<int*>(m + o_bstart);
+  P.blk_order = reinterpret_cast<int*>(m + o_order);
+  if (vrow) P.vrow = reinterpret_cast<int*>(m + o_vrow);
   P.device_bytes = total;
 
   e = hipMemsetAsync(P.mem, 0, b_packed + b_vals + b_srow, s);
@@ -708,9 +742,25 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
                                                  A.nnz, P.round_off);
     e = hipGetLastError();
   }
+  // First slab of every round (host prefix over the rounds' slab counts).
+  std::vector<int> roff_h((size_t)nrounds + 1), rbase_h((size_t)nrounds + 1);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(roff_h.data(), P.round_off, roff_h.size() * sizeof(int),
+                       hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) {
+    long long slabs = 0;
+    for (int r = 0; r < nrounds; ++r) {
+      rbase_h[r] = (int)slabs;
+      slabs += (roff_h[r + 1] - roff_h[r] + kBlkSlab - 1) / kBlkSlab;
+    }
+    rbase_h[nrounds] = (int)slabs;
+    e = hipMemcpyAsync(P.round_base, rbase_h.data(), rbase_h.size() * sizeof(int),
+                       hipMemcpyHostToDevice, s);
+  }
   // the bucket starts become the scatter cursors
-  uint32_t* pk = pairs ? reinterpret_cast<uint32_t*>(work) : P.packed;
-  void* vv = pairs ? work + b_work : P.values;
+  uint32_t* pk = reinterpret_cast<uint32_t*>(work);
+  void* vv = work + b_work;
   if (e == hipSuccess && A.nnz > 0) {
     switch (A.vtype) {
       case LOOPS_VALUE_F32:
@@ -729,9 +779,24 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
                                                                      P.srow);
     e = hipGetLastError();
   }
-  if (pairs && e == hipSuccess && A.nnz > 0) {
-    blk_pair_kernel<<<(unsigned)grid, 256, 0, s>>>(pk, static_cast<const uint32_t*>(vv), A.nnz,
-                                                   reinterpret_cast<u32x2*>(P.packed));
+  if (e == hipSuccess && nrounds > 0) {
+    switch (A.vtype) {
+      case LOOPS_VALUE_F32:
+        blk_lane_order_kernel<float><<<(unsigned)nrounds, 256, 0, s>>>(
+            P.round_off, P.round_base, pk, static_cast<const float*>(vv), P.packed,
+            static_cast<float*>(P.values));
+        break;
+      case LOOPS_VALUE_BF16:
+        blk_lane_order_kernel<bf16_t><<<(unsigned)nrounds, 256, 0, s>>>(
+            P.round_off, P.round_base, pk, static_cast<const bf16_t*>(vv), P.packed,
+            static_cast<bf16_t*>(P.values));
+        break;
+      case LOOPS_VALUE_F16:
+        blk_lane_order_kernel<f16_t><<<(unsigned)nrounds, 256, 0, s>>>(
+            P.round_off, P.round_base, pk, static_cast<const f16_t*>(vv), P.packed,
+            static_cast<f16_t*>(P.values));
+        break;
+    }
     e = hipGetLastError();
   }
   const hipError_t e2 = hipStreamSynchronize(s);

@@ -363,6 +363,40 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
   if (base + kBlkSlab < g.n) g.edge_next = (int)(rw[(base + kBlkSlab) >> 1] & 0xffffu);
 }
 
+// Segmented inclusive scan over the workgroup (block_seg_scan's result and
+// order of combination): wavefront scans, then each lane combines the totals
+// of the wavefronts before its own, read with four 16-byte LDS reads.  Two
+// barriers; every lane must call it.
+__device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int* s_f,
+                                               float& pv, int& pf) {
+  static_assert(kBlkTpb / 64 == 8, "eight wavefront totals, two 16-byte reads each");
+  group_seg_scan<64>(v, f);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 63) {
+    s_v[wave] = v;
+    s_f[wave] = f;
+  }
+  __syncthreads();
+  const f32x4 v0 = *reinterpret_cast<const f32x4*>(s_v);
+  const f32x4 v1 = *reinterpret_cast<const f32x4*>(s_v + 4);
+  const i32x4 f0 = *reinterpret_cast<const i32x4*>(s_f);
+  const i32x4 f1 = *reinterpret_cast<const i32x4*>(s_f + 4);
+  const float wv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+  const int wf[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+  pv = 0.f;
+  pf = 0;
+#pragma unroll
+  for (int w = 0; w < 8; ++w) {
+    if (w < wave) {
+      pv = wf[w] ? wv[w] : pv + wv[w];
+      pf |= wf[w];
+    }
+  }
+  if (!f) v = pv + v;
+  f |= pf;
+  __syncthreads();
+}
+
 template <typename V>
 __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     const uint32_t* __restrict__ packed, const V* __restrict__ vals,
@@ -372,9 +406,9 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     const float* __restrict__ x, int S, int W,
     float* __restrict__ partial, long long part_stride) {
   __shared__ float acc[kBlkRows];
-  __shared__ float stage[kBlkRound];
-  __shared__ float s_v[kBlkTpb / 64];
-  __shared__ int s_f[kBlkTpb / 64];
+  __shared__ __align__(16) float stage[kBlkRound];
+  __shared__ __align__(16) float s_v[kBlkTpb / 64];
+  __shared__ __align__(16) int s_f[kBlkTpb / 64];
   const int s = (int)(blockIdx.x % (unsigned)S);
   const int b = blk_order[blockIdx.x / (unsigned)S];
   const int q = b * S + s;  // stream
@@ -419,12 +453,17 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     float v = 0.f, first_sum = 0.f;
     int f = 1, first_row = -1, row = cur.rows[0];
     bool internal = false;
+    // the lane's 8 slots with two 16-byte LDS reads (slots past the round's
+    // end hold stale products and are not used)
+    const f32x4 sa = *reinterpret_cast<const f32x4*>(stage + k0);
+    const f32x4 sb = *reinterpret_cast<const f32x4*>(stage + k0 + 4);
+    const float sv[kBlkIpt] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
     if (m > 0) {
-      v = stage[k0];
+      v = sv[0];
 #pragma unroll
       for (int i = 1; i < kBlkIpt; ++i) {
         if (i < m) {
-          const float p = stage[k0 + i];
+          const float p = sv[i];
           const int ri = cur.rows[i];
           if (ri != row) {
             if (!internal) {
@@ -446,7 +485,7 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     const float v_own = v;
     float pv;
     int pf;
-    block_seg_scan<kBlkTpb>(v, f, s_v, s_f, pv, pf);
+    blk_block_scan(v, f, s_v, s_f, pv, pf);
     // exclusive: the running sum of the previous lane's trailing run
     const float up = __shfl_up(v, 1, 64);
     const float ex = lane == 0 ? pv : up;

@@ -363,14 +363,39 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
   if (base + kBlkSlab < g.n) g.edge_next = (int)(rw[(base + kBlkSlab) >> 1] & 0xffffu);
 }
 
-// Segmented inclusive scan over the workgroup (block_seg_scan's result and
-// order of combination): wavefront scans, then each lane combines the totals
-// of the wavefronts before its own, read with four 16-byte LDS reads.  Two
-// barriers; every lane must call it.
+// Segmented inclusive scan over a wavefront with DPP lane moves: shifts by
+// 1, 2, 4 and 8 inside rows of 16 lanes, then each row takes the total of
+// the rows before it (row_bcast 15 into rows 1 and 3, row_bcast 31 into
+// rows 2 and 3).  A lane whose source is outside the move receives (0, 0),
+// the identity of the combination (v, f) + (v', f') = (f' ? v' : v + v',
+// f | f').
+__device__ __forceinline__ void blk_wave_scan(float& v, int& f) {
+#define BLK_SCAN_STEP(CTRL, ROWS)                                                          \
+  {                                                                                        \
+    const float vu =                                                                       \
+        __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, \
+                                                   false));                                \
+    const int fu = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, false);              \
+    if (!f) v = vu + v;                                                                    \
+    f |= fu;                                                                               \
+  }
+  BLK_SCAN_STEP(0x111, 0xf)  // row_shr:1
+  BLK_SCAN_STEP(0x112, 0xf)  // row_shr:2
+  BLK_SCAN_STEP(0x114, 0xf)  // row_shr:4
+  BLK_SCAN_STEP(0x118, 0xf)  // row_shr:8
+  BLK_SCAN_STEP(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
+  BLK_SCAN_STEP(0x143, 0xc)  // row_bcast:31 into rows 2 and 3
+#undef BLK_SCAN_STEP
+}
+
+// Segmented inclusive scan over the workgroup (block_seg_scan's result):
+// wavefront scans, then each lane combines the totals of the wavefronts
+// before its own, read with four 16-byte LDS reads.  Two barriers; every
+// lane must call it.
 __device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int* s_f,
                                                float& pv, int& pf) {
   static_assert(kBlkTpb / 64 == 8, "eight wavefront totals, two 16-byte reads each");
-  group_seg_scan<64>(v, f);
+  blk_wave_scan(v, f);
   const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 63) {
     s_v[wave] = v;

@@ -524,7 +524,7 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
   }
   __syncthreads();
   float* out = partial + (long long)s * part_stride + r0;
-  for (int i = threadIdx.x; i < nrows; i += kBlkTpb) out[i] = acc[i];
+  for (int i = threadIdx.x; i < nrows; i += kBlkTpb) __builtin_nontemporal_store(acc[i], out + i);
 }
 
 // Sum of the S partials of virtual row v in segment order, loads four at a

@@ -159,3 +159,21 @@ def test_blocks_c2_full_size_auto():
     plan.execute(torch.from_numpy(x_np).cuda(), y)
     torch.cuda.synchronize()
     _check(y.cpu().numpy(), A_h.offsets, A_h.indices, A_h.values, x_np)
+
+
+@pytest.mark.parametrize("shift", [0, 1, 3])
+def test_blocks_ragged_rows_and_unaligned_y(shift):
+    # 1001 rows (the combine's last group of 4 rows is partial) with rows cut
+    # into pieces in the middle and in that last group; y starting `shift`
+    # floats past a 16-byte boundary (shift != 0: the combine's per-row path)
+    off, idx, val = random_csr(1001, 30000, 12, seed=23, long_rows=((500, 9000), (1000, 5000)))
+    A = loops.csr_t.from_host(off.shape[0] - 1, 30000, off, idx, val)
+    plan = loops.spmv_plan(A, layout="blocks")
+    x_np = np.random.default_rng(29).uniform(-1, 1, 30000).astype(np.float32)
+    buf = torch.full((A.rows + 8,), float("nan"), device="cuda")
+    y = buf[shift:shift + A.rows]
+    plan.execute(torch.from_numpy(x_np).cuda(), y)
+    torch.cuda.synchronize()
+    _check(y.cpu().numpy(), off, idx, val, x_np)
+    assert torch.isnan(buf[:shift]).all() and torch.isnan(buf[shift + A.rows:]).all()
+    plan.destroy()

@@ -439,7 +439,6 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
   const int q = b * S + s;  // stream
   const int r0 = blk_start[b];
   const int nrows = blk_start[b + 1] - r0;  // <= kBlkRows
-  for (int i = threadIdx.x; i < nrows; i += kBlkTpb) acc[i] = 0.f;
   const float* __restrict__ xs = x + (long long)s * W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int k0 = threadIdx.x * kBlkIpt;
@@ -447,6 +446,8 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
   blk_round_regs<V> cur;
   if (rb < re)
     blk_load_round<V>(rb, packed, vals, srow, round_off, round_base, lane, wave, k0, cur);
+  // the accumulator is cleared while the first round's loads are in flight
+  for (int i = threadIdx.x; i < nrows; i += kBlkTpb) acc[i] = 0.f;
   for (int r = rb; r < re; ++r) {
     const int n = cur.n;
     // Phase 1: products in column order to their slots; the next round's

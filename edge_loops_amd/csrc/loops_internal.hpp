@@ -109,7 +109,7 @@ struct blk_plan_state {
   // column in segment << kBlkSlotBits | slot in round, and the values.
   uint32_t* packed = nullptr;
   void* values = nullptr;
-  uint16_t* srow = nullptr;     // row in block of slot k of round r at r * kBlkRound + k
+  uint16_t* srow = nullptr;     // stream row of slot k of round r at r * kBlkRound + k
   int* stream_round = nullptr;  // first round of each (block, segment) stream
   int* round_off = nullptr;     // first entry of each round (nrounds + 1), stream order
   int* round_base = nullptr;    // first slab of each round in packed / values (nrounds + 1)
@@ -124,6 +124,15 @@ struct blk_plan_state {
   // cut (virtual row = row).
   int nsplit = 0, vrows = 0;
   int* vrow = nullptr;
+  // Stream rows.  Compressed plans (rlist != null) number, per (block,
+  // segment) stream, only the virtual rows with entries in that segment:
+  // rlist holds them stream by stream (segment-major: segment s, blocks in
+  // order), rstart[s * (nb + 1) + b] where stream (b, s) starts; the LDS
+  // accumulator and the partial writes cover those rows only, and the
+  // partials of the others stay 0 from the build.  Dense plans number all
+  // rows of the block (virtual row - blk_start[b]).
+  int* rstart = nullptr;
+  int* rlist = nullptr;
   void* mem = nullptr;
   size_t device_bytes = 0;
 };

@@ -76,6 +76,11 @@ struct blk_geom {
   int rows, cols, nnz, S, W, nch, nb;  // nch: chunks per segment; nb: row blocks
   const int* blk_start;                // first virtual row of each block (nb + 1)
   const int* vrow;                     // first virtual row of each row, or null
+  // compressed plans: exclusive prefix over (segment, virtual row) pairs,
+  // segment-major, of "the row has entries in the segment" (S V + 1); null
+  // for dense plans
+  const int* excl;
+  long long V;                         // virtual rows
 };
 
 // Virtual row of entry nz of row r: the row itself, or for a row cut into
@@ -113,6 +118,45 @@ __device__ __forceinline__ long long blk_bucket(const blk_geom& g, int b, int co
   const int s = col / g.W;
   const int ch = (col - s * g.W) / kBlkChunkCols;
   return ((long long)b * g.S + s) * g.nch + ch;
+}
+
+// mask[v] |= bit s for every nonzero of virtual row v in segment s (S <= 64).
+__global__ __launch_bounds__(256) void blk_mask_kernel(const int* __restrict__ offsets,
+                                                       const int* __restrict__ indices,
+                                                       blk_geom g,
+                                                       unsigned long long* __restrict__ mask) {
+  for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
+       nz += (long long)gridDim.x * blockDim.x) {
+    const int v = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
+    atomicOr(&mask[v], 1ull << (indices[nz] / g.W));
+  }
+}
+
+// fl[s V + v] = bit s of mask[v].
+__global__ __launch_bounds__(256) void blk_flag_kernel(const unsigned long long* __restrict__ mask,
+                                                       long long V, int S, int* __restrict__ fl) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)S * V;
+       i += (long long)gridDim.x * blockDim.x)
+    fl[i] = (int)((mask[i % V] >> (i / V)) & 1ull);
+}
+
+// rlist[excl[s V + v]] = v for every flagged pair (the stream rows, stream
+// by stream), rstart[s (nb + 1) + b] = excl[s V + blk_start[b]].
+__global__ __launch_bounds__(256) void blk_rlist_kernel(const int* __restrict__ fl,
+                                                        const int* __restrict__ excl, long long V,
+                                                        int S, int* __restrict__ rlist) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)S * V;
+       i += (long long)gridDim.x * blockDim.x)
+    if (fl[i]) rlist[excl[i]] = (int)(i % V);
+}
+__global__ __launch_bounds__(256) void blk_rstart_kernel(const int* __restrict__ excl,
+                                                         const int* __restrict__ blk_start,
+                                                         int nb, long long V, int S,
+                                                         int* __restrict__ rstart) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S * (nb + 1)) return;
+  const int s = t / (nb + 1), b = t % (nb + 1);
+  rstart[t] = excl[(long long)s * V + blk_start[b]];
 }
 
 // cnt[bucket] += 1 for every nonzero.
@@ -210,9 +254,12 @@ __global__ __launch_bounds__(256) void blk_scatter_kernel(
     const int r = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
     const int b = blk_block_of(g, r);
     const int col = indices[nz];
+    const int sg = col / g.W;
     const int pos = atomicAdd(&cursor[blk_bucket(g, b, col)], 1);
-    packed[pos] = (uint32_t)(col - (col / g.W) * g.W);
-    trow[pos] = (uint16_t)(r - g.blk_start[b]);
+    packed[pos] = (uint32_t)(col - sg * g.W);
+    trow[pos] = (uint16_t)(g.excl ? g.excl[(long long)sg * g.V + r] -
+                                        g.excl[(long long)sg * g.V + g.blk_start[b]]
+                                  : r - g.blk_start[b]);
     vals[pos] = values[nz];
   }
 }
@@ -390,11 +437,12 @@ __device__ __forceinline__ void blk_wave_scan(float& v, int& f) {
 
 // Segmented inclusive scan over the workgroup (block_seg_scan's result):
 // wavefront scans, then each lane combines the totals of the wavefronts
-// before its own, read with four 16-byte LDS reads.  Two barriers; every
+// before its own, read with 16-byte LDS reads.  Two barriers; every
 // lane must call it.
 __device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int* s_f,
                                                float& pv, int& pf) {
-  static_assert(kBlkTpb / 64 == 8, "eight wavefront totals, two 16-byte reads each");
+  constexpr int NW = kBlkTpb / 64;
+  static_assert(NW == 4 || NW == 8, "four or eight wavefront totals, 16-byte reads");
   blk_wave_scan(v, f);
   const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 63) {
@@ -402,16 +450,19 @@ __device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int
     s_f[wave] = f;
   }
   __syncthreads();
-  const f32x4 v0 = *reinterpret_cast<const f32x4*>(s_v);
-  const f32x4 v1 = *reinterpret_cast<const f32x4*>(s_v + 4);
-  const i32x4 f0 = *reinterpret_cast<const i32x4*>(s_f);
-  const i32x4 f1 = *reinterpret_cast<const i32x4*>(s_f + 4);
-  const float wv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-  const int wf[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+  float wv[NW];
+  int wf[NW];
+#pragma unroll
+  for (int h = 0; h < NW; h += 4) {
+    const f32x4 vv = *reinterpret_cast<const f32x4*>(s_v + h);
+    const i32x4 ff = *reinterpret_cast<const i32x4*>(s_f + h);
+    wv[h] = vv.x; wv[h + 1] = vv.y; wv[h + 2] = vv.z; wv[h + 3] = vv.w;
+    wf[h] = ff.x; wf[h + 1] = ff.y; wf[h + 2] = ff.z; wf[h + 3] = ff.w;
+  }
   pv = 0.f;
   pf = 0;
 #pragma unroll
-  for (int w = 0; w < 8; ++w) {
+  for (int w = 0; w < NW; ++w) {
     if (w < wave) {
       pv = wf[w] ? wv[w] : pv + wv[w];
       pf |= wf[w];
@@ -428,6 +479,7 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     const uint16_t* __restrict__ srow, const int* __restrict__ stream_round,
     const int* __restrict__ round_off, const int* __restrict__ round_base,
     const int* __restrict__ blk_start, const int* __restrict__ blk_order,
+    const int* __restrict__ rstart, const int* __restrict__ rlist, int nb,
     const float* __restrict__ x, int S, int W,
     float* __restrict__ partial, long long part_stride) {
   __shared__ float acc[kBlkRows];
@@ -437,8 +489,17 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
   const int s = (int)(blockIdx.x % (unsigned)S);
   const int b = blk_order[blockIdx.x / (unsigned)S];
   const int q = b * S + s;  // stream
-  const int r0 = blk_start[b];
-  const int nrows = blk_start[b + 1] - r0;  // <= kBlkRows
+  // stream rows: the compressed list positions rs .. rs + nrows - 1, or the
+  // block's rows r0 .. r0 + nrows - 1 (<= kBlkRows either way)
+  int r0 = 0, rs = 0, nrows;
+  if (rlist) {
+    const int* rp = rstart + (long long)s * (nb + 1) + b;
+    rs = rp[0];
+    nrows = rp[1] - rs;
+  } else {
+    r0 = blk_start[b];
+    nrows = blk_start[b + 1] - r0;
+  }
   const float* __restrict__ xs = x + (long long)s * W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int k0 = threadIdx.x * kBlkIpt;
@@ -524,8 +585,14 @@ __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
     if (r + 1 < re) cur = nxt;
   }
   __syncthreads();
-  float* out = partial + (long long)s * part_stride + r0;
-  for (int i = threadIdx.x; i < nrows; i += kBlkTpb) __builtin_nontemporal_store(acc[i], out + i);
+  float* out = partial + (long long)s * part_stride;
+  if (rlist) {
+    for (int i = threadIdx.x; i < nrows; i += kBlkTpb)
+      __builtin_nontemporal_store(acc[i], out + rlist[rs + i]);
+  } else {
+    for (int i = threadIdx.x; i < nrows; i += kBlkTpb)
+      __builtin_nontemporal_store(acc[i], out + r0 + i);
+  }
 }
 
 // Sum of the S partials of virtual row v in segment order, loads four at a
@@ -578,7 +645,8 @@ template <typename V>
 hipError_t blk_execute(const blk_plan_state& st, const float* x, hipStream_t s) {
   spmv_blocks_kernel<V><<<(unsigned)((long long)st.nb * st.S), kBlkTpb, 0, s>>>(
       st.packed, static_cast<const V*>(st.values), st.srow, st.stream_round, st.round_off,
-      st.round_base, st.blk_start, st.blk_order, x, st.S, st.W, st.partial, st.part_stride);
+      st.round_base, st.blk_start, st.blk_order, st.rstart, st.rlist, st.nb, x, st.S, st.W,
+      st.partial, st.part_stride);
   return hipGetLastError();
 }
 }  // namespace
@@ -601,18 +669,19 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   P.W = (int)((A.cols + P.S - 1) / P.S);
   if (P.W < 1) P.W = 1;
   if (A.rows == 0) return hipSuccess;  // nothing to store; execution only records ev1
-  // Row blocks: at most kBlkRows (virtual) rows and about `target` nonzeros
-  // each, a row closing a block when that overshoots the target less than
-  // stopping short would.  The target aims at two waves of resident
-  // workgroups (two per CU) less 1/32, so that uneven blocks even out and
-  // every XCD keeps a few slots spare (with exactly two waves the dispatcher
-  // left a few workgroups of some XCDs to start a third, 45 us late, on the
-  // 2M-row power-law matrix); it grows until the cut yields no more blocks
-  // than that.  A row longer than a quarter of the target (and
-  // than a round) is cut into pieces, entry i of m to piece i % m, so that
-  // every piece spans the columns as the row does and no block's streams
-  // outgrow the others' (the heaviest blocks run first, the lightest fill
-  // in behind them).
+  // Row blocks: about `target` nonzeros each, a row closing a block when
+  // that overshoots the target less than stopping short would, and no
+  // stream holding more than kBlkRows rows.  Compressed plans (at most 64
+  // segments, and rows with entries in at most 3/4 of the (segment, row)
+  // pairs) count a row only in the segments where it has entries; dense
+  // plans count every row in every segment.  The cut aims at one wave of
+  // big blocks (resident workgroups, two per CU, less 1/32 so every XCD
+  // keeps a few slots spare; big blocks put more of a stream's entries on
+  // each x line) when the row capacity allows; a one-wave cut more uneven
+  // than 1.2 x is redone for two waves.  The target grows until the cut yields no more
+  // blocks than wanted.  A row longer than an eighth of the one-wave share
+  // (and than a round) is cut into pieces, entry i of m to piece i % m, so
+  // that every piece spans the columns as the row does.
   std::vector<int> off_h((size_t)A.rows + 1, 0), starts(1, 0);
   hipError_t e = hipMemcpyAsync(off_h.data(), A.offsets, ((size_t)A.rows + 1) * sizeof(int),
                                 hipMemcpyDeviceToHost, s);
@@ -621,11 +690,10 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   if (e == hipSuccess) e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  long long want = ((long long)A.rows + kBlkRows - 1) / kBlkRows;
-  const long long by_cus = (2LL * 2 * cus * 31 / 32) / P.S;
-  if (want < by_cus) want = by_cus;
-  long long target = A.nnz / want > 0 ? A.nnz / want : 1;
-  const long long cap = target / 4 > kBlkRound ? target / 4 : kBlkRound;
+  const long long wgs = 2LL * cus * 31 / 32;
+  const long long want1 = wgs / P.S > 0 ? wgs / P.S : 1;  // blocks of one wave
+  const long long share1 = A.nnz / want1 > 0 ? A.nnz / want1 : 1;
+  const long long cap = share1 / 8 > kBlkRound ? share1 / 8 : kBlkRound;
   std::vector<int> vrow_h;
   long long V = A.rows;
   for (int r = 0; r < A.rows; ++r) {
@@ -647,33 +715,106 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
     }
     vrow_h[A.rows] = v;
   }
-  std::vector<long long> bnnz;
-  for (int tries = 0;; ++tries) {
-    starts.assign(1, 0);
-    bnnz.clear();
-    int rows_in = 0, v = 0;
-    long long nnz_in = 0;
-    auto add = [&](long long L) {
-      if (rows_in > 0 && (rows_in == kBlkRows ||
-                          (nnz_in + L > target && nnz_in + L - target > target - nnz_in))) {
-        starts.push_back(v);
-        bnnz.push_back(nnz_in);
-        rows_in = 0;
-        nnz_in = 0;
-      }
-      ++rows_in;
-      nnz_in += L;
-      ++v;
-    };
-    for (int r = 0; r < A.rows; ++r) {
-      const long long L = off_h[r + 1] - off_h[r];
-      const long long m = L > cap ? (L + cap - 1) / cap : 1;
-      for (long long i = 0; i < m; ++i) add((L - i + m - 1) / m);
+  const bool masks = P.S <= 64 && (long long)P.S * V <= (1LL << 28);
+
+  // Early scratch, kept to the end: first virtual row of each row, segment
+  // masks of the virtual rows (at most 64 segments).
+  const size_t b_vrow = al256(vrow_h.size() * sizeof(int));
+  const size_t b_mask = masks ? al256((size_t)V * sizeof(unsigned long long)) : 0;
+  char* sc0 = nullptr;
+  if ((e = hipMalloc(&sc0, b_vrow + b_mask + 256)) != hipSuccess) return e;
+  int* vrow = P.nsplit > 0 ? reinterpret_cast<int*>(sc0) : nullptr;
+  unsigned long long* mask = masks ? reinterpret_cast<unsigned long long*>(sc0 + b_vrow) : nullptr;
+  long long grid = ((long long)A.nnz + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  if (grid < 1) grid = 1;
+  std::vector<unsigned long long> mask_h;
+  if (vrow)
+    e = hipMemcpyAsync(vrow, vrow_h.data(), vrow_h.size() * sizeof(int), hipMemcpyHostToDevice,
+                       s);
+  if (masks && e == hipSuccess) {
+    e = hipMemsetAsync(mask, 0, (size_t)V * sizeof(unsigned long long), s);
+    const blk_geom g0{A.rows, A.cols, A.nnz, P.S, P.W, 0, 0, nullptr, vrow, nullptr, V};
+    if (e == hipSuccess && A.nnz > 0) {
+      blk_mask_kernel<<<(unsigned)grid, 256, 0, s>>>(A.offsets, A.indices, g0, mask);
+      e = hipGetLastError();
     }
-    starts.push_back(v);
-    bnnz.push_back(nnz_in);
-    if ((long long)bnnz.size() <= want || tries == 63) break;
-    target += target / 64 + 1;
+    mask_h.resize((size_t)V);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(mask_h.data(), mask, (size_t)V * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(sc0);
+    return e;
+  }
+
+  // Compressed only where it relaxes the row capacity: when rows have
+  // entries in at most 3/4 of the (segment, row) pairs.
+  std::vector<long long> rows_in_seg((size_t)P.S, 0);
+  long long flagged = 0;
+  for (unsigned long long mm : mask_h)
+    for (; mm; mm &= mm - 1) ++rows_in_seg[__builtin_ctzll(mm)], ++flagged;
+  const bool comp = masks && flagged * 4 <= 3LL * P.S * V;
+  std::vector<long long> bnnz;
+  std::vector<int> cs((size_t)P.S);
+  auto cut = [&](long long want) {
+    long long target = A.nnz / want > 0 ? A.nnz / want : 1;
+    for (int tries = 0;; ++tries) {
+      starts.assign(1, 0);
+      bnnz.clear();
+      std::fill(cs.begin(), cs.end(), 0);
+      int rows_in = 0, v = 0;
+      long long nnz_in = 0;
+      auto add = [&](long long L) {
+        bool full = rows_in == kBlkRows;
+        if (comp) {
+          full = false;
+          for (unsigned long long mm = mask_h[v]; mm; mm &= mm - 1)
+            full = full || cs[__builtin_ctzll(mm)] == kBlkRows;
+        }
+        if (rows_in > 0 &&
+            (full || (nnz_in + L > target && nnz_in + L - target > target - nnz_in))) {
+          starts.push_back(v);
+          bnnz.push_back(nnz_in);
+          rows_in = 0;
+          nnz_in = 0;
+          std::fill(cs.begin(), cs.end(), 0);
+        }
+        if (comp)
+          for (unsigned long long mm = mask_h[v]; mm; mm &= mm - 1) ++cs[__builtin_ctzll(mm)];
+        ++rows_in;
+        nnz_in += L;
+        ++v;
+      };
+      for (int r = 0; r < A.rows; ++r) {
+        const long long L = off_h[r + 1] - off_h[r];
+        const long long m = L > cap ? (L + cap - 1) / cap : 1;
+        for (long long i = 0; i < m; ++i) add((L - i + m - 1) / m);
+      }
+      starts.push_back(v);
+      bnnz.push_back(nnz_in);
+      if ((long long)bnnz.size() <= want || tries == 63) break;
+      target += target / 64 + 1;
+    }
+  };
+  {
+    // blocks the row capacity needs (compressed: the segment with the most
+    // rows); one wave when that fits and the cut is even, else two
+    long long need = (V + kBlkRows - 1) / kBlkRows;
+    if (comp) {
+      need = 1;
+      for (long long c : rows_in_seg) need = std::max(need, (c + kBlkRows - 1) / kBlkRows);
+    }
+    const long long slots = 2LL * cus / P.S;  // blocks of one full wave
+    long long want = std::max(want1, need);
+    if (want > slots) want = std::max(2 * want1, need);
+    cut(want);
+    long long most = 0;
+    for (long long n : bnnz) most = n > most ? n : most;
+    if (want <= slots && (double)most * bnnz.size() > 1.2 * (double)A.nnz)
+      cut(std::max(2 * want1, need));
   }
   P.nb = (int)starts.size() - 1;
   // Launch order: heaviest blocks first.
@@ -684,55 +825,82 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const int nch = (P.W + kBlkChunkCols - 1) / kBlkChunkCols;
   const long long nstreams = (long long)P.nb * P.S;
   const long long nbuckets = nstreams * nch;
-  if (nbuckets + 1 >= (long long)INT32_MAX) return hipErrorInvalidValue;
+  if (nbuckets + 1 >= (long long)INT32_MAX) {
+    (void)hipFree(sc0);
+    return hipErrorInvalidValue;
+  }
   const int vbytes = A.vtype == LOOPS_VALUE_F32 ? 4 : 2;
   P.part_stride = (V + 63) & ~63LL;
 
   // Build scratch: bucket counts, bucket starts (+1), rounds per stream and
-  // their exclusive prefix (+1), scan scratch.
+  // their exclusive prefix (+1), scan scratch, rows of entries in stream
+  // order, block starts, (compressed) pair flags and their exclusive prefix.
+  const long long npairs = comp ? (long long)P.S * V : 0;
+  long long nscan = nbuckets > nstreams ? nbuckets : nstreams;
+  if (npairs > nscan) nscan = npairs;
   const size_t b_cnt = al256((size_t)nbuckets * sizeof(int));
   const size_t b_start = al256((size_t)(nbuckets + 1) * sizeof(int));
   const size_t b_nr = al256((size_t)(nstreams + 1) * sizeof(int));
-  const size_t b_scan = al256(exclusive_scan_scratch_bytes(nbuckets > nstreams ? nbuckets
-                                                                               : nstreams));
+  const size_t b_scan = al256(exclusive_scan_scratch_bytes(nscan));
   const size_t b_trow = al256(((size_t)A.nnz + 8) * sizeof(uint16_t));
   const size_t b_bstart = al256(((size_t)P.nb + 1) * sizeof(int));
   const size_t b_order = al256(((size_t)P.nb + 1) * sizeof(int));
-  const size_t b_vrow = al256(vrow_h.size() * sizeof(int));
+  const size_t b_fl = comp ? al256((size_t)npairs * sizeof(int)) : 0;
+  const size_t b_excl = comp ? al256((size_t)(npairs + 1) * sizeof(int)) : 0;
   // Entries in stream order (column | slot words, values) until they are
   // stored lane-major in the plan.
   const size_t n_pad = (size_t)A.nnz + 8;
   const size_t b_work = al256(n_pad * sizeof(uint32_t));
   const size_t b_workv = al256(n_pad * vbytes);
+  size_t sc_bytes = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t at = sc_bytes;
+    sc_bytes += bytes;
+    return at;
+  };
+  const size_t c_cnt = carve(b_cnt), c_start = carve(b_start), c_nr = carve(b_nr),
+               c_first = carve(b_nr), c_scan = carve(b_scan), c_trow = carve(b_trow),
+               c_bstart = carve(b_bstart), c_fl = carve(b_fl), c_excl = carve(b_excl),
+               c_work = carve(b_work), c_workv = carve(b_workv);
+  (void)c_workv;
   char* sc = nullptr;
-  e = hipMalloc(&sc, b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow + b_work +
-                         b_workv);
-  if (e != hipSuccess) return e;
-  int* cnt = reinterpret_cast<int*>(sc);
-  int* start = reinterpret_cast<int*>(sc + b_cnt);
-  int* nr = reinterpret_cast<int*>(sc + b_cnt + b_start);
-  int* first = reinterpret_cast<int*>(sc + b_cnt + b_start + b_nr);
-  void* scan_scratch = sc + b_cnt + b_start + 2 * b_nr;
-  uint16_t* trow = reinterpret_cast<uint16_t*>(sc + b_cnt + b_start + 2 * b_nr + b_scan);
-  int* bstart = reinterpret_cast<int*>(sc + b_cnt + b_start + 2 * b_nr + b_scan + b_trow);
-  int* vrow = P.nsplit > 0 ? reinterpret_cast<int*>(sc + b_cnt + b_start + 2 * b_nr + b_scan +
-                                                    b_trow + b_bstart)
-                           : nullptr;
-  char* work = sc + b_cnt + b_start + 2 * b_nr + b_scan + b_trow + b_bstart + b_vrow;
-  const blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart, vrow};
-  long long grid = ((long long)A.nnz + 255) / 256;
-  if (grid > 65536) grid = 65536;
-  if (grid < 1) grid = 1;
+  if ((e = hipMalloc(&sc, sc_bytes)) != hipSuccess) {
+    (void)hipFree(sc0);
+    return e;
+  }
+  int* cnt = reinterpret_cast<int*>(sc + c_cnt);
+  int* start = reinterpret_cast<int*>(sc + c_start);
+  int* nr = reinterpret_cast<int*>(sc + c_nr);
+  int* first = reinterpret_cast<int*>(sc + c_first);
+  void* scan_scratch = sc + c_scan;
+  uint16_t* trow = reinterpret_cast<uint16_t*>(sc + c_trow);
+  int* bstart = reinterpret_cast<int*>(sc + c_bstart);
+  int* fl = comp ? reinterpret_cast<int*>(sc + c_fl) : nullptr;
+  int* excl = comp ? reinterpret_cast<int*>(sc + c_excl) : nullptr;
+  char* work = sc + c_work;
+  const blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart, vrow, excl, V};
   const unsigned sgrid = (unsigned)((nstreams + 3) / 4 > 0 ? (nstreams + 3) / 4 : 1);  // wave per stream
+  auto fail = [&](hipError_t err) {
+    (void)hipFree(sc);
+    (void)hipFree(sc0);
+    return err;
+  };
 
-  // Plan memory, part 1: stream -> first round (needed to count rounds).
+  // Plan memory, part 1: stream -> first round (needed to count rounds);
+  // compressed plans: the pair flags, their prefix and its total.
   const size_t b_sround = al256((size_t)(nstreams + 1) * sizeof(int));
-  int nrounds = 0;
+  int nrounds = 0, np = 0;
   e = hipMemcpyAsync(bstart, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice,
                      s);
-  if (e == hipSuccess && vrow)
-    e = hipMemcpyAsync(vrow, vrow_h.data(), vrow_h.size() * sizeof(int), hipMemcpyHostToDevice,
-                       s);
+  if (comp && e == hipSuccess) {
+    long long fgrid = (npairs + 255) / 256;
+    if (fgrid > 65536) fgrid = 65536;
+    blk_flag_kernel<<<(unsigned)fgrid, 256, 0, s>>>(mask, V, P.S, fl);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_exclusive_scan(fl, npairs, excl, scan_scratch, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&np, excl + npairs, sizeof(int), hipMemcpyDeviceToHost, s);
+  }
   if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, b_cnt, s);
   if (e == hipSuccess && A.nnz > 0) {
     blk_count_kernel<<<(unsigned)grid, 256, 0, s>>>(A.offsets, A.indices, g, cnt);
@@ -748,10 +916,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   if (e == hipSuccess)
     e = hipMemcpyAsync(&nrounds, first + nstreams, sizeof(int), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    (void)hipFree(sc);
-    return e;
-  }
+  if (e != hipSuccess) return fail(e);
   P.nrounds = nrounds;
 
   // Plan memory, part 2: entries lane-major in whole slabs per round (at
@@ -773,10 +938,9 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const size_t o_sround = take(b_sround), o_roff = take(b_roff), o_rbase = take(b_roff);
   const size_t o_part = take(b_part), o_bstart = take(b_bstart), o_order = take(b_order);
   const size_t o_vrow = take(b_vrow);
-  if ((e = hipMalloc(&P.mem, total)) != hipSuccess) {
-    (void)hipFree(sc);
-    return e;
-  }
+  const size_t o_rstart = take(comp ? al256((size_t)P.S * (P.nb + 1) * sizeof(int)) : 0);
+  const size_t o_rlist = take(comp ? al256(((size_t)np + 1) * sizeof(int)) : 0);
+  if ((e = hipMalloc(&P.mem, total)) != hipSuccess) return fail(e);
   char* m = static_cast<char*>(P.mem);
   P.packed = reinterpret_cast<uint32_t*>(m + o_packed);
   P.values = m + o_vals;
@@ -788,9 +952,28 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   P.blk_start = reinterpret_cast<int*>(m + o_bstart);
   P.blk_order = reinterpret_cast<int*>(m + o_order);
   if (vrow) P.vrow = reinterpret_cast<int*>(m + o_vrow);
+  if (comp) {
+    P.rstart = reinterpret_cast<int*>(m + o_rstart);
+    P.rlist = reinterpret_cast<int*>(m + o_rlist);
+  }
   P.device_bytes = total;
 
   e = hipMemsetAsync(P.mem, 0, b_packed + b_vals + b_srow, s);
+  // compressed plans write the partials of their stream rows only: the
+  // others stay 0
+  if (comp && e == hipSuccess) e = hipMemsetAsync(P.partial, 0, b_part, s);
+  if (comp && e == hipSuccess) {
+    const int t = P.S * (P.nb + 1);
+    blk_rstart_kernel<<<(unsigned)((t + 255) / 256), 256, 0, s>>>(excl, bstart, P.nb, V, P.S,
+                                                                 P.rstart);
+    e = hipGetLastError();
+    long long fgrid = (npairs + 255) / 256;
+    if (fgrid > 65536) fgrid = 65536;
+    if (e == hipSuccess) {
+      blk_rlist_kernel<<<(unsigned)fgrid, 256, 0, s>>>(fl, excl, V, P.S, P.rlist);
+      e = hipGetLastError();
+    }
+  }
   if (e == hipSuccess)
     e = hipMemcpyAsync(P.stream_round, first, (size_t)(nstreams + 1) * sizeof(int),
                        hipMemcpyDeviceToDevice, s);
@@ -866,6 +1049,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   }
   const hipError_t e2 = hipStreamSynchronize(s);
   (void)hipFree(sc);
+  (void)hipFree(sc0);
   return e != hipSuccess ? e : e2;
 }
 

@@ -128,7 +128,11 @@ __global__ __launch_bounds__(256) void blk_mask_kernel(const int* __restrict__ o
   for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
        nz += (long long)gridDim.x * blockDim.x) {
     const int v = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
-    atomicOr(&mask[v], 1ull << (indices[nz] / g.W));
+    const unsigned long long bit = 1ull << (indices[nz] / g.W);
+    // bits are only ever set: most entries of a long row find theirs set
+    // already and skip the atomic
+    if (!(__hip_atomic_load(&mask[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
+      atomicOr(&mask[v], bit);
   }
 }
 
@@ -761,6 +765,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   std::vector<int> cs((size_t)P.S);
   auto cut = [&](long long want) {
     long long target = A.nnz / want > 0 ? A.nnz / want : 1;
+    size_t last = 0, stalled = 0;
     for (int tries = 0;; ++tries) {
       starts.assign(1, 0);
       bnnz.clear();
@@ -796,6 +801,11 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
       starts.push_back(v);
       bnnz.push_back(nnz_in);
       if ((long long)bnnz.size() <= want || tries == 63) break;
+      // the row capacity, not the target, sets the count: stop when it no
+      // longer falls
+      stalled = bnnz.size() < last ? 0 : stalled + 1;
+      last = bnnz.size();
+      if (stalled == 3) break;
       target += target / 64 + 1;
     }
   };

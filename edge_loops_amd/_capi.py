@@ -194,6 +194,8 @@ SIGNATURES = {
     "loops_spmv_plan_execute": (_st, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "loops_spmv_plan_info": (_st, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i64)]),
     "loops_spmv_plan_layout": (_st, [_vp, ctypes.POINTER(_i32)]),
+    "loops_spmv_plan_blocks": (_st, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                     ctypes.POINTER(_i32)]),
     "loops_spmv_plan_destroy": (_st, [_vp]),
     "loops_batch_query": (_st, [ctypes.POINTER(BatchItem), _i64, ctypes.POINTER(BatchInfo)]),
     "loops_batch_prepare": (_st, [ctypes.POINTER(BatchItem), _i64, _vp, ctypes.c_size_t, _vp,

@@ -192,6 +192,12 @@ class spmv_plan:
         _capi.check(_capi.lib().loops_spmv_plan_layout(h, ctypes.byref(lay)),
                     "loops_spmv_plan_layout")
         self.layout = {v: k for k, v in _capi.PLAN_LAYOUTS.items()}[int(lay.value)]
+        nb, comp, ns = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _capi.check(_capi.lib().loops_spmv_plan_blocks(h, ctypes.byref(nb), ctypes.byref(comp),
+                                                       ctypes.byref(ns)),
+                    "loops_spmv_plan_blocks")
+        # row blocks layout: blocks, compressed stream rows, rows cut into pieces
+        self.blocks, self.compressed, self.split_rows = int(nb.value), bool(comp.value), int(ns.value)
 
     def execute(self, x, y, stream=None, start_event=None, stop_event=None):
         _capi.check(_capi.lib().loops_spmv_plan_execute(

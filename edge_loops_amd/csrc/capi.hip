@@ -382,6 +382,16 @@ loops_status_t loops_spmv_plan_layout(loops_spmv_plan_t plan, int32_t* layout) {
   return LOOPS_SUCCESS;
 }
 
+loops_status_t loops_spmv_plan_blocks(loops_spmv_plan_t plan, int32_t* blocks,
+                                      int32_t* compressed, int32_t* split_rows) {
+  if (!plan) return LOOPS_ERROR_INVALID_VALUE;
+  const bool blk = plan->st.layout == LOOPS_PLAN_LAYOUT_BLOCKS;
+  if (blocks) *blocks = blk ? plan->st.blk.nb : 0;
+  if (compressed) *compressed = blk && plan->st.blk.rlist != nullptr ? 1 : 0;
+  if (split_rows) *split_rows = blk ? plan->st.blk.nsplit : 0;
+  return LOOPS_SUCCESS;
+}
+
 loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan) {
   if (!plan) return LOOPS_ERROR_INVALID_VALUE;
   blk_plan_release(&plan->st.blk);

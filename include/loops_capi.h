@@ -215,6 +215,13 @@ loops_status_t loops_spmv_plan_execute(loops_spmv_plan_t plan, const float* x,
 loops_status_t loops_spmv_plan_info(loops_spmv_plan_t plan, int32_t* panels,
                                     int64_t* device_bytes);
 loops_status_t loops_spmv_plan_layout(loops_spmv_plan_t plan, int32_t* layout);
+/* BLOCKS plans (0 for PANELS; any pointer may be null): blocks, the row
+ * blocks (each runs as one workgroup per column segment); compressed, 1
+ * when a row holds an LDS slot and a partial only in the segments where it
+ * has entries, 0 when it does in every segment of its block; split_rows,
+ * the rows cut into pieces. */
+loops_status_t loops_spmv_plan_blocks(loops_spmv_plan_t plan, int32_t* blocks,
+                                      int32_t* compressed, int32_t* split_rows);
 loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
 
 /* ---- Batches: many independent products in one persistent launch -------

@@ -149,12 +149,34 @@ def test_blocks_graph_replays():
         _check(y.cpu().numpy(), off, idx, val, x_np)
 
 
+def test_blocks_row_modes():
+    # rows of 0 or 1 entries: a row lives in at most one of the 4 segments,
+    # so stream rows are compressed; the narrow dense matrix's rows have
+    # entries in every segment (dense)
+    for name, comp in (("one_entry_rows", True), ("dense_narrow", False),
+                       ("mixed_lengths_4seg", True)):
+        off, idx, val, cols = MATS[name]
+        A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+        plan = loops.spmv_plan(A, layout="blocks")
+        assert plan.compressed == comp, name
+        assert plan.blocks >= 1
+        plan.destroy()
+    off, idx, val, cols = MATS["long_misaligned"]
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+    plan = loops.spmv_plan(A, layout="blocks")
+    assert plan.split_rows == 3  # the rows of 9001, 7003 and 4099 entries (over 4096)
+    plan.destroy()
+
+
 def test_blocks_c2_full_size_auto():
     A_h = synth.c2_powerlaw_2m()
     x_np = synth.uniform(A_h.cols, seed=21)
     A = loops.csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values)
     plan = loops.spmv_plan(A)
     assert plan.layout == "blocks" and plan.panels == 4
+    # one wave of compressed blocks on a 256-CU MI355X (two workgroups per
+    # CU, four segments: at most 128 blocks)
+    assert plan.compressed and plan.blocks <= 2 * torch.cuda.get_device_properties(0).multi_processor_count // 4
     y = torch.empty(A_h.rows, device="cuda")
     plan.execute(torch.from_numpy(x_np).cuda(), y)
     torch.cuda.synchronize()

@@ -2,9 +2,9 @@
 blocks of 15872 rows x column segments (4, or a multiple of 8 for more than
 2^22 columns), each (block, segment) stream bucketed by 64-column chunks
 and cut into rounds of at most 4096 entries sorted by (row, column); y
-accumulated in LDS one round at a time, the segments' partial rows added in
-segment order; one-entry rows written directly by the workgroup of their
-column segment (y starts as NaN, so a row nobody writes fails).
+accumulated in LDS one round at a time; the segments' partial rows (and a
+split row's pieces) added in a fixed order by the combine kernel (y starts
+as NaN, so a row nobody writes fails).
 
 Bar: per row |y - y64| <= 1e-5 * sum_j |a_ij x_j| against the FP64 oracle and
 the reference validator's count of |y - y32| > 1e-2 equal to 0
@@ -56,6 +56,10 @@ def _mats():
     idx = rng.integers(0, 700000, int(lens.sum()))
     m["one_entry_rows"] = (off, idx.astype(np.int32),
                            rng.uniform(-1, 1, idx.shape[0]).astype(np.float32), 700000)
+    # one row holding most of the nonzeros: its pieces fill whole blocks and
+    # span block boundaries
+    off, idx, val = random_csr(30000, 600000, 4, seed=31, long_rows=((777, 400000),))
+    m["row_over_many_blocks"] = (off, idx, val, 600000)
     # one-entry, empty and kept rows interleaved, 4 and 8 column segments
     for cols, name in ((700000, "mixed_lengths_4seg"), (5_000_000, "mixed_lengths_8seg")):
         lens = np.tile(np.array([1, 2, 0, 1, 1, 5, 1, 17], np.int64), 4000)

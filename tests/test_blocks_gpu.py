@@ -47,8 +47,8 @@ def _mats():
     # 64-column chunks with far more than 4096 entries per block: split rounds
     off, idx, val = random_csr(20000, 40, 40, seed=13, empty_frac=0.05)
     m["dense_narrow"] = (off, idx, val, 40)
-    # rows of 0 or 1 entries only: blocks without kept rows, every value
-    # written by the one-entry pass
+    # rows of 0 or 1 entries only: each row a stream row of one segment
+    # (compressed stream rows)
     rng = np.random.default_rng(17)
     lens = rng.integers(0, 2, 30000)
     off = np.zeros(lens.shape[0] + 1, np.int32)
@@ -60,6 +60,22 @@ def _mats():
     # span block boundaries
     off, idx, val = random_csr(30000, 600000, 4, seed=31, long_rows=((777, 400000),))
     m["row_over_many_blocks"] = (off, idx, val, 600000)
+    # every row in segments 1-3 (dense stream rows), every 300th also in
+    # segment 0: that segment's rounds skip 300 rows between slots, the
+    # others step by one
+    r = np.arange(124000)
+    has0 = r % 300 == 0
+    lens = 3 + has0.astype(np.int64)
+    off = np.zeros(r.shape[0] + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    idx = np.empty(int(off[-1]), np.int32)
+    at = off[:-1]
+    idx[at[has0]] = (r % 100)[has0]
+    at = at + has0
+    for u, mul in enumerate((7, 11, 13)):
+        idx[at + u] = 150000 * (u + 1) + (r * mul) % 1000
+    m["gapped_segment_rows"] = (off.astype(np.int32), idx,
+                                rng.uniform(-1, 1, idx.shape[0]).astype(np.float32), 600000)
     # one-entry, empty and kept rows interleaved, 4 and 8 column segments
     for cols, name in ((700000, "mixed_lengths_4seg"), (5_000_000, "mixed_lengths_8seg")):
         lens = np.tile(np.array([1, 2, 0, 1, 1, 5, 1, 17], np.int64), 4000)
@@ -158,7 +174,7 @@ def test_blocks_row_modes():
     # so stream rows are compressed; the narrow dense matrix's rows have
     # entries in every segment (dense)
     for name, comp in (("one_entry_rows", True), ("dense_narrow", False),
-                       ("mixed_lengths_4seg", True)):
+                       ("mixed_lengths_4seg", True), ("gapped_segment_rows", False)):
         off, idx, val, cols = MATS[name]
         A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
         plan = loops.spmv_plan(A, layout="blocks")

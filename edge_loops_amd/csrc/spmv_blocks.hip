@@ -441,8 +441,10 @@ __device__ __forceinline__ void blk_wave_scan(float& v, int& f) {
 
 // Segmented inclusive scan over the workgroup (block_seg_scan's result):
 // wavefront scans, then each lane combines the totals of the wavefronts
-// before its own, read with 16-byte LDS reads.  Two barriers; every
-// lane must call it.
+// before its own, read with 16-byte LDS reads.  One barrier; every lane
+// must call it.  The totals are next written in the following round's scan,
+// after that round's phase-1 barrier, which every wave reaches only once it
+// has read them here.
 __device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int* s_f,
                                                float& pv, int& pf) {
   constexpr int NW = kBlkTpb / 64;
@@ -474,7 +476,6 @@ __device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int
   }
   if (!f) v = pv + v;
   f |= pf;
-  __syncthreads();
 }
 
 template <typename V>

@@ -76,7 +76,8 @@ def _mats():
         idx[at + u] = 150000 * (u + 1) + (r * mul) % 1000
     m["gapped_segment_rows"] = (off.astype(np.int32), idx,
                                 rng.uniform(-1, 1, idx.shape[0]).astype(np.float32), 600000)
-    # one-entry, empty and kept rows interleaved, 4 and 8 column segments
+    # rows of one, none and several entries interleaved, 4 and 8 column
+    # segments
     for cols, name in ((700000, "mixed_lengths_4seg"), (5_000_000, "mixed_lengths_8seg")):
         lens = np.tile(np.array([1, 2, 0, 1, 1, 5, 1, 17], np.int64), 4000)
         off = np.zeros(lens.shape[0] + 1, np.int32)

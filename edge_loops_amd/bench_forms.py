@@ -63,7 +63,7 @@ def batched(count=512, reps=10, table=None, log=print):
     byts = sum(synth.spmv_bytes(m.rows, m.cols, m.nnz) for m in mats)
     flops = 2 * nnz
     table = table or tune.Table.load()
-    picks = [table.pick(m.rows, m.nnz, m.max_row(), 1) for m in mats]
+    picks = table.pick_batch([(m.rows, m.nnz, m.max_row(), 1) for m in mats])
     hist = {}
     for s, i in picks:
         hist[f"{s}:{i}"] = hist.get(f"{s}:{i}", 0) + 1
@@ -79,18 +79,27 @@ def batched(count=512, reps=10, table=None, log=print):
                 "gbs": byts / (ms * 1e-3) / 1e9,
                 "roofline_frac": byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
-    # batched, tuned and fixed
-    results = {}
+    # batched, tuned and fixed: both captured, then timed alternately three
+    # times (the median of each), so neither leg pays for the clocks ramping
+    legs = {}
     for name, pk in (("batched_tuned", picks),
                      ("batched_merge8", [("merge_path_flat", 8)] * count)):
         b = Batch([BatchItem(A, x, y, s, i) for A, x, y, (s, i) in zip(dev, xs, ys, pk)])
-        g = b.capture()
-        ms = _time(g.replay, reps)
-        results[name] = [y.clone() for y in ys] if name == "batched_merge8" else None
+        legs[name] = (b, b.capture())
+    times = {name: [] for name in legs}
+    for _ in range(3):
+        for name, (b, g) in legs.items():
+            times[name].append(_time(g.replay, reps))
+    for name, (b, g) in legs.items():
+        ms = float(np.median(times[name]))
         out[name] = leg(ms)
         out[name]["work_items"] = int(b.info.work_items)
         log(f"{name}: {ms:.3f} ms/pass")
-        del g, b
+    results = {}
+    legs["batched_merge8"][1].replay()
+    torch.cuda.synchronize()
+    results["batched_merge8"] = [y.clone() for y in ys]
+    del legs
 
     # plain per-call
     wss = [torch.empty(max(spmv_workspace_size("merge_path_flat", A, threads_per_block=256,

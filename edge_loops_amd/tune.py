@@ -34,6 +34,13 @@ DEFAULT_SPMM = ("merge_path_flat", 8)
 # faster: smaller differences are within run-to-run noise, and in a mixed
 # batch every extra tile class costs a persistent kernel of its own.
 PICK_MARGIN = 0.05
+# In one batch launch every kernel class (SpMV: merge tiles of 4 steps with
+# the row / lane-group pieces, 8, 16; SpMM: steps per range) is a persistent
+# kernel of its own, run after the others.  A class other than the default's
+# holding less than this share of the batch's work runs with the default
+# instead: one 16-step matrix among 511 8-step ones cost 13-17 us of a
+# 1.28 ms pass.
+MIN_CLASS_SHARE = 0.1
 TABLE_PATH = os.path.join(os.path.dirname(__file__), "tuning", "mi355x_batch.json")
 
 
@@ -71,6 +78,16 @@ def spmv_candidates():
             + [("thread_mapped", 0), ("group_mapped", "x1"), ("group_mapped", "x4")])
 
 
+def kernel_class(sched, ipt, k=1):
+    """The batch kernel an item runs in: ("spmv", 4 | 8 | 16) or ("spmm",
+    steps per range)."""
+    if k > 1:
+        return ("spmm", ipt)
+    if sched == "merge_path_flat" and ipt in (8, 16):
+        return ("spmv", ipt)
+    return ("spmv", 4)
+
+
 def resolve(sched, tile, rows, nnz):
     """items_per_thread for the batch C ABI from a (schedule, tile) name."""
     if sched == "group_mapped":
@@ -105,6 +122,28 @@ class Table:
         """(schedule, items_per_thread) for one product."""
         sched, tile = self.pick_name(rows, nnz, max_row, k)
         return sched, resolve(sched, tile, rows, nnz)
+
+    def pick_batch(self, products):
+        """Picks for the products of one batch launch, products = [(rows,
+        nnz, max_row, k)]: each product's pick, except where its kernel
+        class holds less than MIN_CLASS_SHARE of the batch's work ((rows +
+        nnz) * k) and is not the default's class; those take the default."""
+        picks = [self.pick(r, n, mr, k) for r, n, mr, k in products]
+        work = [(r + n) * max(k, 1) for r, n, _, k in products]
+        share = {}
+        for (sched, ipt), w, (_, _, _, k) in zip(picks, work, products):
+            c = kernel_class(sched, ipt, k)
+            share[c] = share.get(c, 0) + w
+        total = sum(work) or 1
+        out = []
+        for (sched, ipt), (r, n, _, k) in zip(picks, products):
+            d = DEFAULT_SPMV if k == 1 else DEFAULT_SPMM
+            d = (d[0], resolve(d[0], d[1], r, n))
+            c = kernel_class(sched, ipt, k)
+            if c != kernel_class(d[0], d[1], k) and share[c] < MIN_CLASS_SHARE * total:
+                sched, ipt = d
+            out.append((sched, ipt))
+        return out
 
     def pick_name(self, rows, nnz, max_row, k=1):
         key = features(rows, nnz, max_row, k)

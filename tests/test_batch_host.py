@@ -79,6 +79,31 @@ def test_tune_features_and_lookup(tmp_path):
     assert tune.Table.load(str(p)).pick(1000, 12000, 400) == ("merge_path_flat", 16)
 
 
+def test_tune_batch_folds_small_classes():
+    # a bucket whose measured winner is 16 steps per lane
+    t = tune.Table()
+    t.data["spmv"]["k0/s6/d3/w2"] = {"schedule": "merge_path_flat", "tile": "16"}
+    t.data["spmv"]["k0/s9/d0/w0"] = {"schedule": "merge_path_flat", "tile": "8"}
+    t.data["spmm"]["k1/s6/d3/w2"] = {"schedule": "merge_path_flat", "tile": "4"}
+    one = (1000, 12000, 400, 1)        # 16-step bucket, 13,000 work
+    other = (200000, 200000, 1, 1)     # 8-step bucket, 400,000 work
+    # one 16-step product among 20 8-step ones: 0.16 % of the work
+    picks = t.pick_batch([one] + [other] * 20)
+    assert picks == [tune.DEFAULT_SPMV] * 21
+    # as much work in the 16-step class as in the default's: kept
+    picks = t.pick_batch([one] * 31 + [other])
+    assert picks[0] == ("merge_path_flat", 16) and picks[-1] == tune.DEFAULT_SPMV
+    # SpMM classes are counted with their k: 2 x 13,000 of 426,000 folds,
+    # 8 x 13,000 of 504,000 does not
+    assert t.pick(1000, 12000, 400, 2) == ("merge_path_flat", 4)
+    assert t.pick_batch([(1000, 12000, 400, 2), other])[0] == tune.DEFAULT_SPMM
+    assert t.pick_batch([(1000, 12000, 400, 8), other])[0] == ("merge_path_flat", 4)
+    # a small class that is the default's own class stays as picked
+    assert t.pick_batch([other, one])[0] == tune.DEFAULT_SPMV
+    assert tune.kernel_class("group_mapped", 8) == ("spmv", 4)
+    assert tune.kernel_class("merge_path_flat", 16, 32) == ("spmm", 16)
+
+
 def test_committed_table_is_complete():
     t = tune.Table.load()
     assert t.data["spmv"] and t.data["spmm"]

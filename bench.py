@@ -199,6 +199,40 @@ def main():
         extras["plain_per_call_gflops"] = round(flops / (plain_ms * 1e-3) / 1e9, 2)
         log(f"plain per-call loops_spmv: {plain_ms*1e3:.1f} us/step")
 
+        # measured HBM ceilings beside the 8 TB/s peak: a 1 GiB device copy
+        # (bytes read + written) and a read-only reduction over the same GiB
+        big = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").uniform_()
+        dst = torch.empty_like(big)
+
+        def timed(fn, reps=10):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = Event(), Event()
+            e0.record(stream)
+            for _ in range(reps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            return e0.elapsed_ms(e1) / reps
+
+        copy_ms = timed(lambda: dst.copy_(big))
+        read_ms = timed(lambda: torch.sum(big))
+        gib = 2 ** 30
+        props = torch.cuda.get_device_properties(0)
+        extras["device"] = {"name": props.name,
+                            "arch": getattr(props, "gcnArchName", ""),
+                            "compute_units": props.multi_processor_count,
+                            "memory_gib": round(props.total_memory / gib, 1)}
+        extras["hbm_ceilings"] = {
+            "copy_gbs": round(2 * gib / (copy_ms * 1e-3) / 1e9, 1),
+            "read_gbs": round(gib / (read_ms * 1e-3) / 1e9, 1),
+            "what": "torch copy_ of 1 GiB fp32 (bytes read + written) and torch.sum "
+                    "over it (bytes read), HIP events on the bench stream"}
+        log(f"ceilings: copy {extras['hbm_ceilings']['copy_gbs']} GB/s, "
+            f"read {extras['hbm_ceilings']['read_gbs']} GB/s")
+        del big, dst
+
         # all five schedules through loops_spmv, same matrix (copy 0)
         names = ["merge_path_flat", "work_oriented", "group_mapped", "thread_mapped", "original"]
         if args.schedules == "none":

@@ -510,13 +510,59 @@ loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
 }
 }  // namespace
 
+namespace {
+loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
+                                  loops_batch_info_t* info, batch_item_dev* out_table,
+                                  char* dev_ws);
+}  // namespace
+
 // Sizes of the table and of the device workspace for these items; with
 // out_table, also fills the table (host memory) pointing into dev_ws.  The
 // table holds the header, one descriptor per item and BP_COUNT prefix
 // arrays of n + 1 counts: O(n) bytes whatever the sizes of the items.
+// SpMM items of more than kSpmmChunk columns become one item per column
+// chunk (spmm_chunk_width), as loops_spmm launches them (each item
+// validated whole first).
 loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
                             loops_batch_info_t* info, batch_item_dev* out_table,
                             char* dev_ws) {
+  long long pieces = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    item_counts c;
+    const loops_status_t st = count_item(items[i], &c);
+    if (st != LOOPS_SUCCESS) return st;
+    if (items[i].k <= kSpmmChunk) {
+      ++pieces;
+      continue;
+    }
+    for (long long j = 0; j < items[i].k; j += spmm_chunk_width(items[i].k, j)) ++pieces;
+  }
+  if (pieces == n) return batch_layout_items(items, n, info, out_table, dev_ws);
+  if (pieces >= (long long)INT32_MAX) return LOOPS_ERROR_OVERFLOW;
+  auto part_k = [](long long k, long long j) { return (int64_t)spmm_chunk_width(k, j); };
+  std::vector<loops_batch_item_t> split;
+  split.reserve((size_t)pieces);
+  for (int64_t i = 0; i < n; ++i) {
+    const loops_batch_item_t& b = items[i];
+    if (b.k <= kSpmmChunk) {
+      split.push_back(b);
+      continue;
+    }
+    for (int64_t j = 0; j < b.k; j += part_k(b.k, j)) {
+      loops_batch_item_t part = b;
+      part.k = part_k(b.k, j);
+      part.x = b.x + j;
+      part.y = b.y + j;
+      split.push_back(part);
+    }
+  }
+  return batch_layout_items(split.data(), (int64_t)split.size(), info, out_table, dev_ws);
+}
+
+namespace {
+loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
+                                  loops_batch_info_t* info, batch_item_dev* out_table,
+                                  char* dev_ws) {
   long long n_spmv_work = 0, n_spmm_work = 0, n_tiles = 0, n_coords = 0, n_units = 0;
   int max_ipt = 0;
   size_t spmm_ws_bytes = 0;
@@ -651,6 +697,7 @@ loops_status_t batch_layout(const loops_batch_item_t* items, int64_t n,
   P[BP_UNIT][n] = unit;
   return LOOPS_SUCCESS;
 }
+}  // namespace
 
 namespace {
 using spmm_kernel_fn = void (*)(const char*);

@@ -460,7 +460,7 @@ loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
       int tpb = params->threads_per_block ? params->threads_per_block : 256;
       int ipw = params->items_per_thread ? params->items_per_thread : 0;
       if (!mp_spmm_supported(tpb, ipw)) return LOOPS_ERROR_NOT_SUPPORTED;
-      *bytes = mp_spmm_ws_bytes(mp_spmm_tiles(a, (int)k, ipw), (int)k);
+      *bytes = mp_spmm_ws_need(a, (int)k, ipw);
       return LOOPS_SUCCESS;
     }
     default:
@@ -497,7 +497,7 @@ loops_status_t loops_spmm(loops_handle_t handle, loops_schedule_t schedule,
       int tpb = p->threads_per_block ? p->threads_per_block : 256;
       int ipw = p->items_per_thread ? p->items_per_thread : 0;
       if (!mp_spmm_supported(tpb, ipw)) return LOOPS_ERROR_NOT_SUPPORTED;
-      size_t need = mp_spmm_ws_bytes(mp_spmm_tiles(a, (int)k, ipw), (int)k);
+      size_t need = mp_spmm_ws_need(a, (int)k, ipw);
       void* ws = nullptr;
       st = get_workspace(handle, need, workspace, workspace_bytes, &ws);
       if (st != LOOPS_SUCCESS) return st;

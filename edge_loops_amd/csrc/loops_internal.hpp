@@ -183,8 +183,26 @@ hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ld
                           float* C, long long ldc, int tpb, hipEvent_t ev0,
                           hipEvent_t ev1, hipStream_t s);
 bool mp_spmm_supported(int tpb, int ipt);
+// Merge-path SpMM with more than kSpmmChunk columns runs as launches of
+// kSpmmChunk columns (the last one the rest), one after the other on the
+// stream: ranges of 8 lanes x 4 columns with 8 steps beat the wider shapes
+// (1M-row, 16M-nonzero Zipf(2.0) matrix, fp32: k = 64 1.53 -> 1.27 ms,
+// k = 48 1.56 -> 1.10, k = 128 3.84 -> 2.53).  Batches split such items the
+// same way, so an item's result stays the one loops_spmm gives.
+constexpr int kSpmmChunk = 32;
+// Width of the chunk starting at column j of a k-column product (k >
+// kSpmmChunk): kSpmmChunk, except that 33 columns left go as 28 + 5, never
+// leaving a one-column chunk (a one-column batch item is an SpMV, which
+// reads x and y as vectors) and keeping chunk starts 16-byte aligned.
+inline int spmm_chunk_width(long long k, long long j) {
+  const long long left = k - j;
+  if (left == kSpmmChunk + 1) return kSpmmChunk - 4;
+  return left < kSpmmChunk ? (int)left : kSpmmChunk;
+}
 long long mp_spmm_tiles(const csr_args& A, int k, int ipt);
 size_t mp_spmm_ws_bytes(long long tiles, int k);
+// Workspace of a loops_spmm merge-path call: the most any of its chunks needs.
+size_t mp_spmm_ws_need(const csr_args& A, int k, int ipt);
 hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ldb,
                           float* C, long long ldc, int tpb, int ipw, void* ws,
                           hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);

@@ -19,6 +19,8 @@
 //                  finished by a fixed-order fix-up (as for SpMV), per column.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <loops/schedule.hxx>
 #include <loops/util/search.hxx>
 
@@ -303,6 +305,19 @@ size_t mp_spmm_ws_bytes(long long tiles, int k) {
   return tiles > 0 && k > 0 ? spmm_ws_bytes_impl(tiles, k) : 0;
 }
 
+size_t mp_spmm_ws_need(const csr_args& A, int k, int ipt) {
+  if (k <= kSpmmChunk) return mp_spmm_ws_bytes(mp_spmm_tiles(A, k, ipt), k);
+  size_t most = 0;
+  int prev = 0;
+  for (long long j = 0; j < k;) {
+    const int w = spmm_chunk_width(k, j);
+    if (w != prev) most = std::max(most, mp_spmm_ws_bytes(mp_spmm_tiles(A, w, ipt), w));
+    prev = w;
+    j += w;
+  }
+  return most;
+}
+
 hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ldb,
                           float* C, long long ldc, int tpb, int ipt, void* ws,
                           hipEvent_t ev0, hipEvent_t ev1, hipStream_t s) {
@@ -310,14 +325,25 @@ hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ld
   hipError_t e = record(ev0, s);
   if (e != hipSuccess) return e;
   if ((long long)A.rows + A.nnz == 0 || k == 0 || A.rows == 0) return record(ev1, s);
-  const spmm_shape sh = tile_shape(k, vec4_ok(B, ldb, C, ldc));
-  const int it = ipt ? ipt : default_ipt(sh);
-  switch (A.vtype) {
-    case LOOPS_VALUE_F32: return mp_spmm_v<float>(A, k, B, ldb, C, ldc, sh, it, ws, s, ev1);
-    case LOOPS_VALUE_BF16: return mp_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, it, ws, s, ev1);
-    case LOOPS_VALUE_F16: return mp_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, it, ws, s, ev1);
+  // column chunks (spmm_chunk_width), in order; the stop event after the
+  // last chunk's main kernel
+  for (int j = 0, w = 0; j < k; j += w) {
+    w = k > kSpmmChunk ? spmm_chunk_width(k, j) : k;
+    const bool last = j + w >= k;
+    const float* Bj = B + j;
+    float* Cj = C + j;
+    const spmm_shape sh = tile_shape(w, vec4_ok(Bj, ldb, Cj, ldc));
+    const int it = ipt ? ipt : default_ipt(sh);
+    hipEvent_t stop = last ? ev1 : nullptr;
+    switch (A.vtype) {
+      case LOOPS_VALUE_F32: e = mp_spmm_v<float>(A, w, Bj, ldb, Cj, ldc, sh, it, ws, s, stop); break;
+      case LOOPS_VALUE_BF16: e = mp_spmm_v<bf16_t>(A, w, Bj, ldb, Cj, ldc, sh, it, ws, s, stop); break;
+      case LOOPS_VALUE_F16: e = mp_spmm_v<f16_t>(A, w, Bj, ldb, Cj, ldc, sh, it, ws, s, stop); break;
+      default: e = hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
   }
-  return hipErrorInvalidValue;
+  return hipSuccess;
 }
 
 }  // namespace loops_mi355x

@@ -376,10 +376,12 @@ loops_status_t loops_server_replay(loops_server_t server,
  * (container/matrix.cuh:27-35) with leading dimensions ldb, ldc >= k.
  * Schedules: THREAD_MAPPED (algorithms/spmm/thread_mapped.cuh:28-53),
  * GROUP_MAPPED and MERGE_PATH_FLAT (new in this library; the reference has
- * only thread_mapped SpMM).  MERGE_PATH_FLAT: params->items_per_thread merge
- * steps per lane range (4, 8 or 16; 0 = 16 when a range spans 16 lanes or
- * more, i.e. k above 32 with 16-byte aligned B and C rows or k above 8
- * otherwise, else 8). */
+ * only thread_mapped SpMM).  MERGE_PATH_FLAT: more than 32 columns run as
+ * launches of 32 columns each, in order on the stream (33 columns left go
+ * as 28 + 5); params->items_per_thread merge steps per lane range in each
+ * (4, 8 or 16; 0 = 16 when a range spans 16 lanes or more, i.e. above 8
+ * columns without 16-byte aligned B and C rows, else 8); the workspace
+ * size covers the widest launch. */
 loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
                                          loops_schedule_t schedule,
                                          const loops_csr_view_t* A, int64_t k,

@@ -41,6 +41,19 @@ def test_layout_counts():
     # k = 32 with 16-byte rows: 4 columns per lane, 8 lanes -> 256-step tiles at ipt 8
     st, info = query([item(1000, 9000, k=32, ipt=16)])
     assert st == _capi.SUCCESS and info.spmm_units == 20
+    # k = 72: chunks of 32, 32 and 8 columns (as loops_spmm launches them):
+    # 40 + 40 units of 256 steps (8 lanes x 4 columns, 8 steps) and 40 for
+    # the 8 columns (8 lanes x 1 column, 8 steps)
+    st, info = query([item(1000, 9000, k=72, ldx=72, ldy=72)])
+    assert st == _capi.SUCCESS
+    assert (info.spmm_units, info.coordinates) == (120, 123)
+    # k = 33: 28 + 5 columns, never a one-column chunk (that would be an
+    # SpMV item): 40 units of 256 steps, then 5 columns (8 lanes x 1 column)
+    st, info = query([item(1000, 9000, k=33, ldx=36, ldy=36)])
+    assert st == _capi.SUCCESS and info.spmm_units == 80 and info.spmv_tiles == 0
+    # a wide item is validated whole: ldx < k is rejected even though each
+    # chunk would fit
+    assert query([item(1000, 9000, k=72, ldx=40)])[0] == _capi.ERROR_INVALID_VALUE
     st, info = query([item(1000, 9000, ipt=4), item(1000, 9000, k=8), item(50, 0, ipt=16)])
     assert st == _capi.SUCCESS
     assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 41 + 2

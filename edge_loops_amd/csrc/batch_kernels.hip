@@ -378,7 +378,8 @@ __global__ __launch_bounds__(256) void batch_spmv_fixup_kernel(const char* __res
 }
 
 // SpMM tile fix-up across the batch: one lane per global tile slot probes
-// its tile; its wavefront then finishes the rows found, one after another.
+// its tile; a row with one tile of carries is finished by that lane, the
+// others by its wavefront, one after another.
 __global__ __launch_bounds__(256) void batch_spmm_fixup_kernel(const char* __restrict__ table) {
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
@@ -393,7 +394,10 @@ __global__ __launch_bounds__(256) void batch_spmm_fixup_kernel(const char* __res
     t = s - P[item];
     r = spmm_fixup_probe(it.offsets, it.rows, it.ti, t, it.sx, &t_end);
   }
-  unsigned long long todo = __ballot(r >= 0);
+  const bool alone = r >= 0 && t_end == t + 1;
+  if (alone) spmm_fixup_short(items[item].k, t, t_end, r, items[item].sx, items[item].y,
+                              items[item].ldy);
+  unsigned long long todo = __ballot(r >= 0 && !alone);
   while (todo) {
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1;

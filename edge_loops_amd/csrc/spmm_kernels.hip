@@ -121,7 +121,11 @@ __global__ __launch_bounds__(256) void spmm_fixup_kernel(const int* __restrict__
   const long long t = t0 + (threadIdx.x & 63);
   long long t_end = 0;
   const int r = t < tiles ? spmm_fixup_probe(offsets, rows, ti, t, ws, &t_end) : -1;
-  unsigned long long todo = __ballot(r >= 0);
+  // rows with one tile of carries by their own lane, the others by the
+  // wavefront, one after another
+  const bool alone = r >= 0 && t_end == t + 1;
+  if (alone) spmm_fixup_short(k, t, t_end, r, ws, C, ldc);
+  unsigned long long todo = __ballot(r >= 0 && !alone);
   while (todo) {
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1;

@@ -122,6 +122,7 @@ PLAN_LAYOUTS = {"auto": 0, "panels": 1, "blocks": 2}
 SERVER_BATCHED = 0
 SERVER_PLAIN = 1
 SERVER_FLAG_REFERENCE_SCHEDULES = 1
+SERVER_FLAG_BATCH_ALL = 2
 
 
 class ServerConfig(ctypes.Structure):

@@ -177,10 +177,10 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
            "pool_rows": int(sum(m.rows for m in mats)), "requests": n_requests,
            "mean_k": float(ks.mean()), "slots": slots}
 
-    def server(mode, vt, plain_slots=64, reference=False, warm=200):
+    def server(mode, vt, plain_slots=64, reference=False, warm=200, batch_all=False):
         srv = Server(mode=mode, arena_bytes=32 << 30, scratch_bytes=1 << 30,
                      slots=slots if mode == "batched" else plain_slots,
-                     reference_schedules=reference)
+                     reference_schedules=reference, batch_all=batch_all)
         dev, tun = pools[vt]
         ids = [srv.add_matrix(A, tunings=t) for A, t in zip(dev, tun)]
         reqs = [(ids[m], x1[m] if k == 1 else X32[m][:, :k]) for m, k in zip(mid, ks)]
@@ -234,6 +234,13 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
         log(f"batched {vt}: capacity {cap:.0f} req/s; at {lam0:.0f}/s p99 "
             f"{res[f'batched_{vt}_at_load']['p99_ms']:.2f} ms; at {hi:.0f}/s p99 "
             f"{res[f'batched_{vt}_at_3x_load']['p99_ms']:.2f} ms")
+    # every request in a batch (none launched alone): the batch kernels'
+    # own capacity on these large requests
+    srv, reqs = server("batched", "f32", batch_all=True)
+    res["batched_all_f32_burst"] = run(srv, reqs, None)
+    srv.close()
+    log(f"batched f32, every request in a batch: capacity "
+        f"{res['batched_all_f32_burst']['requests_per_s']:.0f} req/s")
     out.update(res)
     out["capacity_ratio_f32"] = res["batched_f32_burst"]["requests_per_s"] / cap_plain
     out["capacity_ratio_f32_vs_blocking"] = (res["batched_f32_burst"]["requests_per_s"] /

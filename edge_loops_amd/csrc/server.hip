@@ -104,6 +104,7 @@ struct request_t {
   size_t y_off = 0, y_bytes = 0;
   float* y = nullptr;
   size_t ws_need = 0, table_need = 0;  // bounds of its share of a batch
+  bool direct = false;  // BATCHED mode: launched alone ahead of its batch
   double t_submit = 0.0, t_done = 0.0;
   int state = RQ_QUEUED;
 };
@@ -148,7 +149,7 @@ struct loops_server {
   arena_t arena;
   std::vector<loops_server_matrix_t> matrices;  // guarded by mu
   std::vector<slot_t> slots;
-  loops_handle_t handle = nullptr;  // plain mode
+  loops_handle_t handle = nullptr;  // per-call launches
   void* plain_ws = nullptr;
   std::mutex mu;
   std::condition_variable cv_done;  // request completed
@@ -170,6 +171,7 @@ struct loops_server {
   bool reference_schedules() const {
     return (cfg.flags & LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES) != 0;
   }
+  bool batch_all() const { return (cfg.flags & LOOPS_SERVER_FLAG_BATCH_ALL) != 0; }
 };
 
 namespace {
@@ -189,38 +191,87 @@ void fail_all_locked(loops_server* S, loops_status_t st, int hip) {
   S->pending.clear();
 }
 
+// BATCHED mode: requests of at least this many work items ((rows + nnz) x
+// k) are launched alone through loops_spmv / loops_spmm on the slot's
+// stream, ahead of the batch of the others.  What a batch saves is launch
+// overhead, small beside such a product (a 2M-nonzero k = 4 product runs
+// ~47 us alone), while the batch adds its prologue (~8 us) and the
+// persistent kernels' item search (~7 % on SpMM).  The result is the same
+// either way (batch items are computed as alone).
+constexpr int64_t kDirectWork = 1 << 20;
+
+// The request's product through the per-call entry points, on `stream`
+// with `ws` as workspace (PLAIN mode, and BATCHED mode's direct requests).
+loops_status_t launch_one(loops_server* S, const request_t* r, void* ws, size_t ws_bytes,
+                          hipStream_t stream) {
+  loops_params_t p{};
+  if (r->k == 1) {
+    loops_schedule_t sched = r->schedule;
+    if (S->reference_schedules()) {
+      sched = LOOPS_SCHEDULE_MERGE_PATH_FLAT;  // library defaults
+    } else if (sched == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
+      p.threads_per_block = 256;
+      p.items_per_thread = r->ipt ? r->ipt : 8;
+    }
+    return loops_spmv(S->handle, sched, &r->m.A, r->x, r->y, &p, ws, ws_bytes, stream);
+  }
+  if (S->reference_schedules())
+    return loops_spmm(S->handle, LOOPS_SCHEDULE_THREAD_MAPPED, &r->m.A, r->k, r->x, r->ldx,
+                      r->y, r->k, &p, ws, ws_bytes, stream);
+  p.items_per_thread = r->ipt;
+  return loops_spmm(S->handle, LOOPS_SCHEDULE_MERGE_PATH_FLAT, &r->m.A, r->k, r->x, r->ldx,
+                    r->y, r->k, &p, ws, ws_bytes, stream);
+}
+
+// Workspace of launch_one for the request.
+loops_status_t one_workspace(loops_server* S, const request_t* r, size_t* need) {
+  loops_params_t p{};
+  if (r->k == 1) {
+    loops_schedule_t sched = r->schedule;
+    if (S->reference_schedules()) {
+      sched = LOOPS_SCHEDULE_MERGE_PATH_FLAT;
+    } else if (sched == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
+      p.threads_per_block = 256;
+      p.items_per_thread = r->ipt ? r->ipt : 8;
+    }
+    return loops_spmv_workspace_size(nullptr, sched, &r->m.A, &p, need);
+  }
+  if (S->reference_schedules())
+    return loops_spmm_workspace_size(nullptr, LOOPS_SCHEDULE_THREAD_MAPPED, &r->m.A, r->k, &p,
+                                     need);
+  p.items_per_thread = r->ipt;
+  return loops_spmm_workspace_size(nullptr, LOOPS_SCHEDULE_MERGE_PATH_FLAT, &r->m.A, r->k, &p,
+                                   need);
+}
+
 // Launches one batch (or, in plain mode, one request) on slot sl.
 loops_status_t launch(loops_server* S, slot_t& sl, const std::vector<request_t*>& batch) {
   if (S->plain()) {
-    request_t* r = batch[0];
-    loops_params_t p{};
-    loops_status_t st;
-    if (r->k == 1) {
-      loops_schedule_t sched = r->schedule;
-      if (S->reference_schedules()) {
-        sched = LOOPS_SCHEDULE_MERGE_PATH_FLAT;  // library defaults
-      } else if (sched == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
-        p.threads_per_block = 256;
-        p.items_per_thread = r->ipt ? r->ipt : 8;
-      }
-      st = loops_spmv(S->handle, sched, &r->m.A, r->x, r->y, &p, S->plain_ws,
-                      (size_t)S->cfg.scratch_bytes, sl.stream);
-    } else if (S->reference_schedules()) {
-      st = loops_spmm(S->handle, LOOPS_SCHEDULE_THREAD_MAPPED, &r->m.A, r->k, r->x, r->ldx,
-                      r->y, r->k, &p, S->plain_ws, (size_t)S->cfg.scratch_bytes, sl.stream);
-    } else {
-      p.items_per_thread = r->ipt;
-      st = loops_spmm(S->handle, LOOPS_SCHEDULE_MERGE_PATH_FLAT, &r->m.A, r->k, r->x,
-                      r->ldx, r->y, r->k, &p, S->plain_ws, (size_t)S->cfg.scratch_bytes,
-                      sl.stream);
-    }
+    const loops_status_t st =
+        launch_one(S, batch[0], S->plain_ws, (size_t)S->cfg.scratch_bytes, sl.stream);
     if (st != LOOPS_SUCCESS) return st;
     return hipEventRecord(sl.done, sl.stream) == hipSuccess ? LOOPS_SUCCESS
                                                             : LOOPS_ERROR_HIP;
   }
-  std::vector<loops_batch_item_t> items(batch.size());
-  for (size_t i = 0; i < batch.size(); ++i) {
-    const request_t* r = batch[i];
+  // direct requests first, each with the slot's scratch as its workspace
+  // (stream order separates their uses and the batch's)
+  std::vector<const request_t*> rest;
+  rest.reserve(batch.size());
+  for (const request_t* r : batch) {
+    if (!r->direct) {
+      rest.push_back(r);
+      continue;
+    }
+    const loops_status_t st =
+        launch_one(S, r, sl.scratch, (size_t)S->cfg.scratch_bytes, sl.stream);
+    if (st != LOOPS_SUCCESS) return st;
+  }
+  if (rest.empty())
+    return hipEventRecord(sl.done, sl.stream) == hipSuccess ? LOOPS_SUCCESS
+                                                            : LOOPS_ERROR_HIP;
+  std::vector<loops_batch_item_t> items(rest.size());
+  for (size_t i = 0; i < rest.size(); ++i) {
+    const request_t* r = rest[i];
     loops_batch_item_t& it = items[i];
     it.A = r->m.A;
     it.k = r->k;
@@ -298,11 +349,16 @@ void server_loop(loops_server* S) {
         std::lock_guard<std::mutex> lk(S->mu);
         if (S->fatal != LOOPS_SUCCESS) break;
         size_t share = 0;
+        int direct = 0;
         while (!S->pending.empty() && batch.size() < cap) {
           request_t* r = S->pending.front();
-          const size_t need = r->ws_need + r->table_need;
+          const size_t need = r->direct ? 0 : r->ws_need + r->table_need;
           if (!S->plain() && !batch.empty() && share + need > (size_t)S->cfg.scratch_bytes)
             break;
+          // one request launched alone per slot: the next waits for a free
+          // slot (and its stream) rather than behind this one
+          if (r->direct && direct == 1) break;
+          direct += r->direct ? 1 : 0;
           size_t off = 0;
           if (!S->arena.alloc(r->y_bytes, &off)) break;  // admission by memory
           r->y_off = off;
@@ -377,8 +433,9 @@ loops_status_t loops_server_create(const loops_server_config_t* config, loops_se
   loops_server_config_t c = config ? *config : loops_server_config_t{};
   if (c.arena_bytes < 0 || c.scratch_bytes < 0 || c.slots < 0 || c.max_batch_requests < 0 ||
       (c.mode != LOOPS_SERVER_BATCHED && c.mode != LOOPS_SERVER_PLAIN) ||
-      (c.flags & ~LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES) != 0 ||
-      (c.flags != 0 && c.mode != LOOPS_SERVER_PLAIN))
+      (c.flags & ~(LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES | LOOPS_SERVER_FLAG_BATCH_ALL)) != 0 ||
+      ((c.flags & LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES) && c.mode != LOOPS_SERVER_PLAIN) ||
+      ((c.flags & LOOPS_SERVER_FLAG_BATCH_ALL) && c.mode != LOOPS_SERVER_BATCHED))
     return LOOPS_ERROR_INVALID_VALUE;
   if (c.arena_bytes == 0) c.arena_bytes = (int64_t)1 << 30;
   if (c.scratch_bytes == 0) c.scratch_bytes = (int64_t)64 << 20;
@@ -411,11 +468,9 @@ loops_status_t loops_server_create(const loops_server_config_t* config, loops_se
         hipMalloc(reinterpret_cast<void**>(&sl.scratch), (size_t)c.scratch_bytes) != hipSuccess)
       return fail(LOOPS_ERROR_ALLOCATION);
   }
-  if (S->plain()) {
-    if (loops_create(&S->handle) != LOOPS_SUCCESS) return fail(LOOPS_ERROR_HIP);
-    if (hipMalloc(&S->plain_ws, (size_t)c.scratch_bytes) != hipSuccess)
-      return fail(LOOPS_ERROR_ALLOCATION);
-  }
+  if (loops_create(&S->handle) != LOOPS_SUCCESS) return fail(LOOPS_ERROR_HIP);
+  if (S->plain() && hipMalloc(&S->plain_ws, (size_t)c.scratch_bytes) != hipSuccess)
+    return fail(LOOPS_ERROR_ALLOCATION);
   S->th = std::thread(server_loop, S);
   *out = S;
   return LOOPS_SUCCESS;
@@ -486,27 +541,16 @@ loops_status_t loops_server_submit(loops_server_t S, int32_t matrix, int64_t k, 
   it.items_per_thread = r->ipt;
   loops_batch_info_t info{};
   loops_status_t st = loops_batch_query(&it, 1, &info);
-  if (st == LOOPS_SUCCESS && S->plain()) {
+  if (st == LOOPS_SUCCESS) {
     size_t need = 0;
-    loops_params_t p{};
-    if (k == 1) {
-      loops_schedule_t sched = r->schedule;
-      if (S->reference_schedules()) {
-        sched = LOOPS_SCHEDULE_MERGE_PATH_FLAT;
-      } else if (sched == LOOPS_SCHEDULE_MERGE_PATH_FLAT) {
-        p.threads_per_block = 256;
-        p.items_per_thread = r->ipt ? r->ipt : 8;
-      }
-      st = loops_spmv_workspace_size(nullptr, sched, &r->m.A, &p, &need);
-    } else if (S->reference_schedules()) {
-      st = loops_spmm_workspace_size(nullptr, LOOPS_SCHEDULE_THREAD_MAPPED, &r->m.A, k, &p,
-                                     &need);
+    const loops_status_t so = one_workspace(S, r, &need);
+    const bool fits = so == LOOPS_SUCCESS && need <= (size_t)S->cfg.scratch_bytes;
+    if (S->plain()) {
+      st = so != LOOPS_SUCCESS ? so : fits ? LOOPS_SUCCESS : LOOPS_ERROR_WORKSPACE;
     } else {
-      p.items_per_thread = r->ipt;
-      st = loops_spmm_workspace_size(nullptr, LOOPS_SCHEDULE_MERGE_PATH_FLAT, &r->m.A, k, &p,
-                                     &need);
+      const int64_t work = ((int64_t)r->m.A.rows + r->m.A.nnzs) * k;
+      r->direct = !S->batch_all() && fits && work >= kDirectWork;
     }
-    if (st == LOOPS_SUCCESS && need > (size_t)S->cfg.scratch_bytes) st = LOOPS_ERROR_WORKSPACE;
   }
   if (st != LOOPS_SUCCESS) {
     delete r;

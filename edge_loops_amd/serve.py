@@ -42,11 +42,14 @@ def matrix_tunings(csr, table=None, max_row=None):
 
 class Server:
     def __init__(self, mode="batched", arena_bytes=0, scratch_bytes=0, slots=0,
-                 max_batch_requests=0, reference_schedules=False):
+                 max_batch_requests=0, reference_schedules=False, batch_all=False):
         """reference_schedules (plain mode only): every request through the
         reference's own entry points (merge_path_flat SpMV, thread_mapped
-        SpMM) instead of the tuned merge-path kernels."""
+        SpMM) instead of the tuned merge-path kernels.  batch_all (batched
+        mode only): every admitted request in the slot's batch, none launched
+        alone."""
         flags = _capi.SERVER_FLAG_REFERENCE_SCHEDULES if reference_schedules else 0
+        flags |= _capi.SERVER_FLAG_BATCH_ALL if batch_all else 0
         cfg = _capi.ServerConfig(int(arena_bytes), int(scratch_bytes), int(slots),
                                  int(max_batch_requests), MODES[mode], flags)
         self.ptr = ctypes.c_void_p()

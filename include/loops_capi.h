@@ -290,8 +290,11 @@ loops_status_t loops_batch_launch(const void* device_table,
  * No counterpart in the reference.  A server owns a device arena for
  * request outputs, `slots` streams each with a batch table and workspace of
  * scratch_bytes, and a thread that admits requests in arrival order once
- * their output fits in the arena and launches everything admitted as one
- * batch (loops_batch_*) on a free slot, so batches overlap on the device.
+ * their output fits in the arena and launches everything admitted on a
+ * free slot, so slots overlap on the device: requests of at least 2^20
+ * work items ((rows + nnzs) x k) alone through loops_spmv / loops_spmm
+ * (their launch overhead is small beside them), the others as one batch
+ * (loops_batch_*).
  * Mode PLAIN launches each request alone through loops_spmv / loops_spmm on
  * one stream instead (the per-call baseline, same admission and clock).
  * Results are bit-identical whatever else is in flight. */
@@ -305,6 +308,8 @@ typedef enum loops_server_mode_t {
  * spmm::thread_mapped (algorithms/spmm/thread_mapped.cuh:67-71, its only
  * SpMM) for k > 1 -- instead of the tuned merge-path kernels. */
 #define LOOPS_SERVER_FLAG_REFERENCE_SCHEDULES 1u
+/* BATCHED mode only: every admitted request in the batch, none alone. */
+#define LOOPS_SERVER_FLAG_BATCH_ALL 2u
 
 typedef struct loops_server_config_t {
   int64_t arena_bytes;        /* outputs of admitted requests; 0 = 1 GiB        */

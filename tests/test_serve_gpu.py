@@ -77,17 +77,22 @@ def test_same_alone_and_in_the_mix(vt):
 
 
 def test_plain_mode_equals_batched():
+    # batched mode launches the large requests alone and batches the rest;
+    # batch_all puts every request in the batch: all three bit-identical
     outs = {}
-    for mode in ("batched", "plain"):
-        with Server(mode=mode) as srv:
+    for name, mode, batch_all in (("batched", "batched", False), ("plain", "plain", False),
+                                  ("batch_all", "batched", True)):
+        with Server(mode=mode, batch_all=batch_all) as srv:
             dev, reqs = _setup(srv, "f32")
             tickets = [srv.submit(mid, x) for mid, m, x, xh in reqs]
-            outs[mode] = [srv.result(t).cpu().numpy() for t in tickets]
+            outs[name] = [srv.result(t).cpu().numpy() for t in tickets]
             if mode == "plain":
                 st = srv.stats()
                 assert st["batches"] == st["completed"] == len(reqs)
-    for a, b in zip(outs["batched"], outs["plain"]):
-        assert np.array_equal(a, b)
+    for a, b, c in zip(outs["batched"], outs["plain"], outs["batch_all"]):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    with pytest.raises(loops.LoopsError):
+        Server(mode="plain", batch_all=True)
 
 
 def test_plain_reference_schedules():

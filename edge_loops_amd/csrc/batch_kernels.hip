@@ -434,11 +434,11 @@ struct spmm_pick {
   int g, cpl;
 };
 inline spmm_pick spmm_shape_for(const loops_batch_item_t& b) {
-  const bool vec4 = b.k >= 16 && b.ldx % 4 == 0 && b.ldy % 4 == 0 &&
-                    reinterpret_cast<uintptr_t>(b.x) % 16 == 0 &&
-                    reinterpret_cast<uintptr_t>(b.y) % 16 == 0;
+  // the rule of loops_spmm (kSpmmVec4MinK): B's alignment only
+  const bool vec4 = b.k >= kSpmmVec4MinK && b.ldx % 4 == 0 &&
+                    reinterpret_cast<uintptr_t>(b.x) % 16 == 0;
   if (vec4) {
-    int g = 4;
+    int g = kSpmmMinG;
     while (g < 64 && g * 4 < b.k) g *= 2;
     return {g, 4};
   }

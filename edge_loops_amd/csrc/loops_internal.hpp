@@ -190,6 +190,15 @@ bool mp_spmm_supported(int tpb, int ipt);
 // k = 48 1.56 -> 1.10, k = 128 3.84 -> 2.53).  Batches split such items the
 // same way, so an item's result stays the one loops_spmm gives.
 constexpr int kSpmmChunk = 32;
+// SpMM lanes take 4 consecutive columns (16-byte B loads) when B's rows are
+// 16-byte aligned (B and ldb) and k is at least this, else 1 column.  C is
+// written with 16-byte stores where its address allows, else column by
+// column: C's alignment does not pick the shape.  400k-row, 2M-nonzero
+// Zipf(2.0) matrix, C rows of k floats: k = 11-15 74-83 us against 100-103
+// with 1 column per lane; k = 4-5 63-70 us with 1 column against 72-76 with
+// 4; k = 8 60 us in ranges of 2 lanes x 4 columns (72 with 1 column or 4
+// lanes).  Ranges hold the fewest lanes (at least 2) that span k columns.
+constexpr int kSpmmVec4MinK = 8;
 // Width of the chunk starting at column j of a k-column product (k >
 // kSpmmChunk): kSpmmChunk, except that 33 columns left go as 28 + 5, never
 // leaving a one-column chunk (a one-column batch item is an SpMV, which

@@ -166,8 +166,8 @@ struct spmm_shape {
   int cpl;
 };
 inline spmm_shape pick_shape(int k, bool vec4) {
-  if (vec4 && k >= 16) {
-    int g = 4;
+  if (vec4 && k >= kSpmmVec4MinK) {
+    int g = kSpmmMinG;
     while (g < 64 && g * 4 < k) g *= 2;
     return {g, 4};
   }
@@ -236,9 +236,9 @@ hipError_t mp_spmm_v(const csr_args& A, int k, const float* B, long long ldb,
   return hipErrorInvalidValue;
 }
 
-inline bool vec4_ok(const float* B, long long ldb, const float* C, long long ldc) {
-  return (reinterpret_cast<uintptr_t>(B) % 16 == 0) && (ldb % 4 == 0) &&
-         (reinterpret_cast<uintptr_t>(C) % 16 == 0) && (ldc % 4 == 0);
+// 16-byte B loads: B's rows 16-byte aligned.
+inline bool vec4_ok(const float* B, long long ldb) {
+  return (reinterpret_cast<uintptr_t>(B) % 16 == 0) && (ldb % 4 == 0);
 }
 
 // Lanes per range of the merge-path SpMM tile (at least kSpmmMinG).
@@ -277,7 +277,7 @@ hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ld
   hipError_t e = record(ev0, s);
   if (e != hipSuccess) return e;
   if (A.rows > 0 && k > 0) {
-    spmm_shape sh = pick_shape(k, vec4_ok(B, ldb, C, ldc));
+    spmm_shape sh = pick_shape(k, vec4_ok(B, ldb));
     switch (A.vtype) {
       case LOOPS_VALUE_F32: e = gm_spmm_v<float>(A, k, B, ldb, C, ldc, sh, s); break;
       case LOOPS_VALUE_BF16: e = gm_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, s); break;
@@ -336,7 +336,7 @@ hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ld
     const bool last = j + w >= k;
     const float* Bj = B + j;
     float* Cj = C + j;
-    const spmm_shape sh = tile_shape(w, vec4_ok(Bj, ldb, Cj, ldc));
+    const spmm_shape sh = tile_shape(w, vec4_ok(Bj, ldb));
     const int it = ipt ? ipt : default_ipt(sh);
     hipEvent_t stop = last ? ev1 : nullptr;
     switch (A.vtype) {

@@ -42,7 +42,9 @@ template <int CPL>
 __device__ __forceinline__ void store_c(float* __restrict__ p, int c, int k,
                                         const float* acc) {
   if constexpr (CPL == 4) {
-    if (c + 4 <= k) {
+    // rows of C (or of the fix-up slots, k floats each) need not be
+    // 16-byte aligned
+    if (c + 4 <= k && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
       f32x4 q;
       q.x = acc[0]; q.y = acc[1]; q.z = acc[2]; q.w = acc[3];
       *reinterpret_cast<f32x4*>(p) = q;

@@ -33,20 +33,24 @@ def test_layout_counts():
     assert st == _capi.SUCCESS and info.work_items == 4 and info.spmv_tiles == 0
     st, info = query([item(1000, 9000, schedule="group_mapped", ipt=16)])
     assert st == _capi.SUCCESS and info.work_items == 63
-    # k = 8: ranges of 8 lanes, 32 ranges x 8 steps = 256 steps per tile
+    # k = 8 with 16-byte B rows: 4 columns per lane, ranges of 2 lanes,
+    # 128 ranges x 8 steps = 1024 steps per tile
     st, info = query([item(1000, 9000, k=8)])
     assert st == _capi.SUCCESS
-    assert (info.spmm_units, info.spmm_work_items, info.work_items) == (40, 40, 40)
-    assert info.coordinates == 41
+    assert (info.spmm_units, info.spmm_work_items, info.work_items) == (10, 10, 10)
+    assert info.coordinates == 11
+    # k = 7: 1 column per lane, ranges of 8 lanes, 32 x 8 = 256 steps per tile
+    st, info = query([item(1000, 9000, k=7, ldx=8)])
+    assert st == _capi.SUCCESS and info.spmm_units == 40
     # k = 32 with 16-byte rows: 4 columns per lane, 8 lanes -> 256-step tiles at ipt 8
     st, info = query([item(1000, 9000, k=32, ipt=16)])
     assert st == _capi.SUCCESS and info.spmm_units == 20
     # k = 72: chunks of 32, 32 and 8 columns (as loops_spmm launches them):
-    # 40 + 40 units of 256 steps (8 lanes x 4 columns, 8 steps) and 40 for
-    # the 8 columns (8 lanes x 1 column, 8 steps)
+    # 40 + 40 units of 256 steps (8 lanes x 4 columns, 8 steps) and 10 of
+    # 1024 steps for the 8 columns (2 lanes x 4 columns, 8 steps)
     st, info = query([item(1000, 9000, k=72, ldx=72, ldy=72)])
     assert st == _capi.SUCCESS
-    assert (info.spmm_units, info.coordinates) == (120, 123)
+    assert (info.spmm_units, info.coordinates) == (90, 93)
     # k = 33: 28 + 5 columns, never a one-column chunk (that would be an
     # SpMV item): 40 units of 256 steps, then 5 columns (8 lanes x 1 column)
     st, info = query([item(1000, 9000, k=33, ldx=36, ldy=36)])
@@ -56,8 +60,8 @@ def test_layout_counts():
     assert query([item(1000, 9000, k=72, ldx=40)])[0] == _capi.ERROR_INVALID_VALUE
     st, info = query([item(1000, 9000, ipt=4), item(1000, 9000, k=8), item(50, 0, ipt=16)])
     assert st == _capi.SUCCESS
-    assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 41 + 2
-    assert info.work_items == 51 and info.max_items_per_thread == 16
+    assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 11 + 2
+    assert info.work_items == 21 and info.max_items_per_thread == 16
     assert info.table_bytes > 0 and info.workspace_bytes > 0
 
 
@@ -140,20 +144,22 @@ def test_table_size_grows_with_items_not_with_their_sizes():
 
 def test_spmm_variant_mask():
     # Variant bit = (value type * 2 + (4 columns per lane)) * 3 + (steps 4/8/16 -> 0/1/2).
-    # With the fake addresses B and C rows are 16-byte aligned, so k >= 16 with
-    # ld a multiple of 4 takes 4 columns per lane.
+    # With the fake addresses B rows are 16-byte aligned, so k >= 8 with ldx a
+    # multiple of 4 takes 4 columns per lane.
     def mask(items):
         st, info = query(items)
         assert st == _capi.SUCCESS
         return info.spmm_variant_mask
     assert mask([item(1000, 9000)]) == 0                                   # SpMV only
-    assert mask([item(1000, 9000, k=8, ipt=8)]) == 1 << 1                  # f32, 1 col, 8
+    assert mask([item(1000, 9000, k=7, ipt=8)]) == 1 << 1                  # f32, 1 col, 8
+    assert mask([item(1000, 9000, k=8, ipt=8)]) == 1 << 4                  # f32, 4 cols, 8
+    assert mask([item(1000, 9000, k=12, ipt=8, ldx=14)]) == 1 << 1         # ldx % 4: 1 col
     assert mask([item(1000, 9000, k=16, ipt=4)]) == 1 << 3                 # f32, 4 cols, 4
     assert mask([item(1000, 9000, k=16, ipt=16, vt=_capi.VALUE_BF16)]) == 1 << 11
     assert mask([item(1000, 9000, k=5, ipt=16, vt=_capi.VALUE_F16)]) == 1 << 14
-    assert mask([item(1000, 9000, k=8, ipt=8), item(1000, 9000, k=8, ipt=8),
-                 item(0, 0, k=8, ipt=4)]) == 1 << 1                        # no pieces: no bit
-    assert mask([item(10, 0, k=8, ipt=4)]) == 1 << 0                       # empty rows are work
+    assert mask([item(1000, 9000, k=7, ipt=8), item(1000, 9000, k=7, ipt=8),
+                 item(0, 0, k=7, ipt=4)]) == 1 << 1                        # no pieces: no bit
+    assert mask([item(10, 0, k=7, ipt=4)]) == 1 << 0                       # empty rows are work
 
 
 def test_pick_margin_keeps_the_default_for_small_gains():

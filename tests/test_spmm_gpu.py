@@ -110,3 +110,29 @@ def test_spmm_low_precision_storage(vt):
         stored = A.values.float().cpu().numpy()
         C64, A64 = oracle.spmm_f64(off, idx, stored, Bh)
         assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30), s
+
+
+@pytest.mark.parametrize("k", [9, 13, 17, 30])
+@pytest.mark.parametrize("schedule", ["merge_path", "group_mapped"])
+def test_spmm_aligned_b_unaligned_c(schedule, k):
+    """B rows 16-byte aligned (ldb = 32) and C rows of k floats: lanes take 4
+    columns of B, and C (and the fix-up slots, k floats per tile) are
+    written column by column where a row is not 16-byte aligned; the
+    merge-path result equals the same product as a batch item bit for bit."""
+    from edge_loops_amd.batch import Batch, BatchItem
+    off, idx, val, cols = MATS["powerlaw_20k"]
+    rows = off.shape[0] - 1
+    A = loops.csr_t.from_host(rows, cols, off, idx, val)
+    Bh = synth.dense(cols, 32, seed=40 + k)
+    B = torch.from_numpy(Bh).cuda()[:, :k]
+    C = torch.full((rows, k), float("nan"), device="cuda")
+    loops.run_spmm(schedule, A, B, C)
+    torch.cuda.synchronize()
+    C64, A64 = oracle.spmm_f64(off, idx, val, np.ascontiguousarray(Bh[:, :k]))
+    got = C.cpu().numpy()
+    assert np.all(np.abs(got - C64) <= REL * A64 + 1e-30)
+    if schedule == "merge_path":
+        Cb = torch.full((rows, k), float("nan"), device="cuda")
+        Batch([BatchItem(A, B, Cb, "merge_path_flat", 0)]).launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(got.view(np.uint32), Cb.cpu().numpy().view(np.uint32))

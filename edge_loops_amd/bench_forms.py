@@ -146,20 +146,18 @@ def _pcts(lat):
             "mean_ms": float(lat.mean())}
 
 
-def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
-    """Served workload: 64 HBM-resident matrices (synth.c5_pool_64), requests
-    with a uniform random matrix and k uniform in [1, 32] fp32 right-hand
-    sides (k == 1: a vector), seeded Poisson arrivals.  Batched server vs the
-    plain per-call mode on the same loop and clock."""
+def _served_setup(n_requests, seed, vts, table, log):
+    """The served pool (synth.c5_pool_64) on the device per value type with
+    its tunings, the right-hand sides, and the seeded request mix."""
     import torch
     from . import tune
-    from .serve import Server, matrix_tunings, poisson_arrivals
+    from .serve import matrix_tunings
     t0 = time.time()
     mats = synth.c5_pool_64()
     log(f"served: generated 64 matrices in {time.time() - t0:.1f}s")
     table = table or tune.Table.load()
     pools = {}
-    for vt in ("f32", "bf16"):
+    for vt in vts:
         dev = [csr_t.from_host(m.rows, m.cols, m.offsets, m.indices, m.values, value_type=vt)
                for m in mats]
         tun = [matrix_tunings(A, table, m.max_row()) for A, m in zip(dev, mats)]
@@ -169,6 +167,34 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
     rng = np.random.default_rng(seed)
     mid = rng.integers(0, len(mats), n_requests)
     ks = rng.integers(1, 33, n_requests)
+    return mats, pools, x1, X32, mid, ks
+
+
+def served_burst(mode="batched", n_requests=3000, seed=55, slots=3, log=print, table=None):
+    """One burst of the served workload (all requests at t = 0) through the
+    server in `mode`, fp32: requests per second.  The leg a kernel trace of
+    the served form runs."""
+    from .serve import Server
+    mats, pools, x1, X32, mid, ks = _served_setup(n_requests, seed, ("f32",), table, log)
+    dev, tun = pools["f32"]
+    srv = Server(mode=mode, arena_bytes=32 << 30, scratch_bytes=1 << 30,
+                 slots=slots if mode == "batched" else 64)
+    ids = [srv.add_matrix(A, tunings=t) for A, t in zip(dev, tun)]
+    reqs = [(ids[m], x1[m] if k == 1 else X32[m][:, :k]) for m, k in zip(mid, ks)]
+    lat, span = srv.replay(reqs, np.zeros(len(reqs)))
+    srv.close()
+    rate = len(reqs) / span
+    log(f"served burst, {mode}: {rate:.0f} req/s")
+    return rate
+
+
+def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
+    """Served workload: 64 HBM-resident matrices (synth.c5_pool_64), requests
+    with a uniform random matrix and k uniform in [1, 32] fp32 right-hand
+    sides (k == 1: a vector), seeded Poisson arrivals.  Batched server vs the
+    plain per-call mode on the same loop and clock."""
+    from .serve import Server, poisson_arrivals
+    mats, pools, x1, X32, mid, ks = _served_setup(n_requests, seed, ("f32", "bf16"), table, log)
     flops = float(sum(2 * mats[m].nnz * k for m, k in zip(mid, ks)))
     out = {"workload": "64 CSR matrices, rows log-uniform [1e4, 4e6], nnz log-uniform "
                        "[1e5, 8e6], Zipf 1.8-2.2; requests: uniform matrix, k uniform "

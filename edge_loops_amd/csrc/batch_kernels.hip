@@ -428,8 +428,9 @@ inline int group_lanes(const loops_batch_item_t& b) {
 inline bool valid_group(int g) {
   return g == 2 || g == 4 || g == 8 || g == 16 || g == 32 || g == 64;
 }
-// SpMM lanes per unit and columns per lane: 4 columns per lane (16-byte
-// loads and stores) when k >= 16 and B / C rows are 16-byte aligned.
+// SpMM lanes per unit and columns per lane: 4 columns per lane (16-byte B
+// loads) when k >= kSpmmVec4MinK and B rows are 16-byte aligned, in ranges
+// of the fewest lanes (at least kSpmmMinG) spanning k; else 1.
 struct spmm_pick {
   int g, cpl;
 };

@@ -293,7 +293,7 @@ bool mp_spmm_supported(int tpb, int ipt) {
 }
 
 // Tiles of the launch: the most over both shapes (4 columns per lane when B
-// and C rows are 16-byte aligned, else 1), so the workspace fits any B / C.
+// rows are 16-byte aligned, else 1), so the workspace fits any B.
 long long mp_spmm_tiles(const csr_args& A, int k, int ipt) {
   long long most = 0;
   for (int v4 = 0; v4 < 2; ++v4) {

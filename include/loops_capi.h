@@ -383,9 +383,11 @@ loops_status_t loops_server_replay(loops_server_t server,
  * GROUP_MAPPED and MERGE_PATH_FLAT (new in this library; the reference has
  * only thread_mapped SpMM).  MERGE_PATH_FLAT: more than 32 columns run as
  * launches of 32 columns each, in order on the stream (33 columns left go
- * as 28 + 5); params->items_per_thread merge steps per lane range in each
- * (4, 8 or 16; 0 = 16 when a range spans 16 lanes or more, i.e. above 8
- * columns without 16-byte aligned B and C rows, else 8); the workspace
+ * as 28 + 5).  Lanes take 4 columns (16-byte B loads) from 8 columns up
+ * when B's rows are 16-byte aligned (B and ldb), else 1; C's alignment does
+ * not matter.  params->items_per_thread merge steps per lane range in each
+ * launch (4, 8 or 16; 0 = 16 when a range spans 16 lanes or more, i.e.
+ * above 8 columns without 16-byte aligned B rows, else 8); the workspace
  * size covers the widest launch. */
 loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
                                          loops_schedule_t schedule,

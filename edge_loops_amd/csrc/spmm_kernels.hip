@@ -212,7 +212,7 @@ hipError_t mp_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
 
 #define LOOPS_SPMM_SHAPES(X) \
   X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(64, 1) \
-  X(4, 4) X(8, 4) X(16, 4) X(32, 4) X(64, 4)
+  X(2, 4) X(4, 4) X(8, 4) X(16, 4) X(32, 4) X(64, 4)
 
 template <typename V>
 hipError_t gm_spmm_v(const csr_args& A, int k, const float* B, long long ldb,

@@ -112,7 +112,7 @@ def test_spmm_low_precision_storage(vt):
         assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30), s
 
 
-@pytest.mark.parametrize("k", [9, 13, 17, 30])
+@pytest.mark.parametrize("k", [8, 9, 13, 17, 30])
 @pytest.mark.parametrize("schedule", ["merge_path", "group_mapped"])
 def test_spmm_aligned_b_unaligned_c(schedule, k):
     """B rows 16-byte aligned (ldb = 32) and C rows of k floats: lanes take 4

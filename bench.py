@@ -324,6 +324,8 @@ def main():
             "kernel": ("spmv_blocks_kernel" if layout == "blocks" else
                        "spmv_panels_kernel" if n_panels > 1 else "spmv_merge_path_kernel"),
             "kernel_avg_ms": round(kern_avg, 5),
+            "kernel_median_ms": round(statistics.median(kern_ms), 5),
+            "kernel_launches": len(kern_ms),
             "algorithmic_bytes": nbytes,
         },
         "cpu_baseline": {

@@ -12,18 +12,20 @@ matrices in one persistent launch vs per-matrix plain launches) and the
 served form (continuous batching over 64 resident matrices vs plain
 per-call launches: req/s, p50/p99), see edge_loops_amd/bench_forms.py.
 
-A step is one execution of the matrix's merge-path SpMV plan
-(loops_spmv_plan_execute: the tiles of 4 column panels of A in one launch,
-the row fix-up, the panel combine), built once per matrix beforehand (its
-build time is reported).  The timed steps rotate over several identical
+A step is one execution of the matrix's SpMV plan (loops_spmv_plan_execute;
+the automatic layout for this matrix is row blocks: one workgroup per (row
+block, column segment) stream accumulating y in LDS, then the combine of the
+segments' partial rows), built once per matrix beforehand (its build time is
+reported; --layout panels selects the column-panel merge-path plan).  The timed steps rotate over several identical
 copies of A (working set > the 256 MB MALL) so no step is served from the
 previous step's cache lines; the same-copy (warm) figure and the plain
 per-call drop-in (loops_spmv, no plan) are reported beside it.
 
 Prints one JSON line (value = GFLOP/s = 2 nnz / step time) with a roofline
 object for the main kernel (algorithmic bytes / its average duration from
-HIP events recorded around it on its stream) and the CPU baseline (the
-oracle's sequential validator loop on the host).
+HIP events recorded around it on its stream), the CPU baseline (the
+oracle's sequential validator loop on the host), the device's properties
+and measured HBM ceilings (a 1 GiB copy and read).
 """
 import argparse
 import json

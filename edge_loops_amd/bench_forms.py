@@ -247,8 +247,45 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
     srv.close()
     log(f"plain: capacity {cap_plain:.0f} req/s; at {lam0:.0f}/s "
         f"p99 {res['plain_f32_at_load']['p99_ms']:.2f} ms")
+    def check(srv, reqs, vt, n_check=24, n_rows=400):
+        """Error of a mixed burst of n_check requests against the FP64 host
+        product, on n_rows sampled rows of each (bf16: on the stored
+        values): rows beyond the 1e-5 per-row bar and the worst relative
+        error."""
+        picks = np.linspace(0, len(reqs) - 1, n_check).astype(int)
+        tickets = [srv.submit(*reqs[i]) for i in picks]
+        bad, worst, checked = 0, 0.0, 0
+        srng = np.random.default_rng(seed + 7)
+        for i, t in zip(picks, tickets):
+            out = srv.result(t)
+            A_h, A = mats[mid[i]], pools[vt][0][mid[i]]
+            rows = np.sort(srng.choice(A_h.rows, min(n_rows, A_h.rows), replace=False))
+            got = out[torch.from_numpy(rows).cuda()].cpu().numpy().astype(np.float64)
+            vals = A.values.float().cpu().numpy() if vt != "f32" else A_h.values
+            lens = A_h.offsets[rows + 1] - A_h.offsets[rows]
+            sub_off = np.zeros(rows.shape[0] + 1, np.int64)
+            np.cumsum(lens, out=sub_off[1:])
+            idx = np.concatenate([A_h.indices[A_h.offsets[r]:A_h.offsets[r + 1]] for r in rows])
+            val = np.concatenate([vals[A_h.offsets[r]:A_h.offsets[r + 1]] for r in rows])
+            X = reqs[i][1].cpu().numpy()
+            r64, a64 = _f64_product(synth.HostCSR(rows.shape[0], A_h.cols, sub_off, idx, val),
+                                    val, X)
+            r64, a64 = r64.reshape(got.shape), a64.reshape(got.shape)
+            err = np.abs(got - r64) / (a64 + 1e-30)
+            bad += int(np.sum(np.any((np.abs(got - r64) > 1e-5 * a64 + 1e-30).reshape(
+                rows.shape[0], -1), axis=1)))
+            worst = max(worst, float(err.max()) if err.size else 0.0)
+            checked += rows.shape[0]
+        return {"requests": int(n_check), "rows_checked": checked, "rows_beyond_1e-5": bad,
+                "worst_rel_err_vs_fp64": worst}
+
+    import torch
     for vt in ("f32", "bf16"):
         srv, reqs = server("batched", vt)
+        res[f"batched_{vt}_check"] = check(srv, reqs, vt)
+        log(f"batched {vt}: {res[f'batched_{vt}_check']['rows_beyond_1e-5']} of "
+            f"{res[f'batched_{vt}_check']['rows_checked']} sampled rows beyond 1e-5 vs FP64, "
+            f"worst {res[f'batched_{vt}_check']['worst_rel_err_vs_fp64']:.2e}")
         res[f"batched_{vt}_burst"] = b = run(srv, reqs, None)
         cap = b["requests_per_s"]
         res[f"batched_{vt}_at_load"] = run(srv, reqs, lam0)

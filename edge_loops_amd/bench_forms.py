@@ -95,6 +95,21 @@ def batched(count=512, reps=10, table=None, log=print):
         out[name] = leg(ms)
         out[name]["work_items"] = int(b.info.work_items)
         log(f"{name}: {ms:.3f} ms/pass")
+    # error of the tuned batch against the FP64 host product, 16 matrices
+    legs["batched_tuned"][1].replay()
+    torch.cuda.synchronize()
+    bad, worst, checked = 0, 0.0, 0
+    for i in np.linspace(0, count - 1, 16).astype(int):
+        m = mats[i]
+        r64, a64 = _f64_product(m, m.values, xs[i].cpu().numpy())
+        got = ys[i].cpu().numpy().astype(np.float64)
+        bad += int(np.sum(np.abs(got - r64) > 1e-5 * a64 + 1e-30))
+        worst = max(worst, float((np.abs(got - r64) / (a64 + 1e-30)).max()))
+        checked += m.rows
+    out["batched_tuned_check"] = {"matrices": 16, "rows_checked": checked,
+                                  "rows_beyond_1e-5": bad, "worst_rel_err_vs_fp64": worst}
+    log(f"batched_tuned: {bad} of {checked} rows of 16 matrices beyond 1e-5 vs FP64, "
+        f"worst {worst:.2e}")
     results = {}
     legs["batched_merge8"][1].replay()
     torch.cuda.synchronize()

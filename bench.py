@@ -309,7 +309,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "c2_powerlaw_2m merge_path_flat SpMV",
+            "workload": f"c2_powerlaw_2m SpMV, loops_spmv_plan ({layout} layout; "
+                        "per-call merge_path_flat and the five schedules beside it)",
             "rows": R, "cols": C, "nnz": N, "max_row": A_h.max_row(),
             "zipf": 1.8, "index": "int32", "values": "f32",
             "plan_layout": layout,

@@ -314,7 +314,7 @@ typedef enum loops_server_mode_t {
 typedef struct loops_server_config_t {
   int64_t arena_bytes;        /* outputs of admitted requests; 0 = 1 GiB        */
   int64_t scratch_bytes;      /* per slot: batch table + workspace; 0 = 64 MiB  */
-  int32_t slots;              /* batches in flight (BATCHED, 0 = 2) or requests
+  int32_t slots;              /* batches in flight (BATCHED, 0 = 3) or requests
                                  in flight (PLAIN, 0 = 64)                      */
   int32_t max_batch_requests; /* 0 = 512                                        */
   int32_t mode;               /* loops_server_mode_t                            */

@@ -306,9 +306,14 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
       p->items_per_thread < 0)
     return LOOPS_ERROR_INVALID_VALUE;
   int layout = p->layout;
+  // AUTO: row blocks from 2^16 columns and one block of rows up (200k x
+  // 200k, 2M nonzeros: 15.6 against 20.2 us with panels; 50k x 50k: a tie),
+  // unless a PANELS parameter is set
   if (layout == LOOPS_PLAN_LAYOUT_AUTO)
-    layout = a.cols >= (1 << 19) && a.rows >= kBlkRows ? LOOPS_PLAN_LAYOUT_BLOCKS
-                                                       : LOOPS_PLAN_LAYOUT_PANELS;
+    layout = a.cols >= (1 << 16) && a.rows >= kBlkRows && p->panels == 0 &&
+                     p->threads_per_block == 0 && p->items_per_thread == 0
+                 ? LOOPS_PLAN_LAYOUT_BLOCKS
+                 : LOOPS_PLAN_LAYOUT_PANELS;
   if (layout == LOOPS_PLAN_LAYOUT_BLOCKS) {
     loops_spmv_plan* pl = new (std::nothrow) loops_spmv_plan();
     if (!pl) return LOOPS_ERROR_ALLOCATION;

@@ -187,8 +187,10 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
 typedef struct loops_spmv_plan* loops_spmv_plan_t;
 
 typedef enum loops_plan_layout_t {
-  LOOPS_PLAN_LAYOUT_AUTO = 0,   /* BLOCKS when cols >= 2^19 and rows >= 15872,
-                                   else PANELS                               */
+  LOOPS_PLAN_LAYOUT_AUTO = 0,   /* BLOCKS when cols >= 2^16, rows >= 15872
+                                   and no PANELS parameter (panels,
+                                   threads_per_block, items_per_thread) is
+                                   set, else PANELS                          */
   LOOPS_PLAN_LAYOUT_PANELS = 1,
   LOOPS_PLAN_LAYOUT_BLOCKS = 2
 } loops_plan_layout_t;

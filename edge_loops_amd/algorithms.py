@@ -168,8 +168,8 @@ class spmv_plan:
     "panels": merge-path tile coordinates computed once and, with panels >
     1, A copied into column panels that run on separate XCDs; "blocks": A
     copied into row blocks x column segments with y accumulated in LDS and
-    x read in column order; "auto": blocks for matrices with at least 2^16
-    columns and 15872 rows when no panels / threads_per_block /
+    x read in column order; "auto": blocks for matrices with at least 2^12
+    columns and 2^19 nonzeros when no panels / threads_per_block /
     items_per_thread is given, else panels.  Both are deterministic.  ``execute(x, y)`` is
     asynchronous and allocates nothing."""
 

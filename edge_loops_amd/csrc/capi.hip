@@ -306,11 +306,13 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
       p->items_per_thread < 0)
     return LOOPS_ERROR_INVALID_VALUE;
   int layout = p->layout;
-  // AUTO: row blocks from 2^16 columns and one block of rows up (200k x
-  // 200k, 2M nonzeros: 15.6 against 20.2 us with panels; 50k x 50k: a tie),
-  // unless a PANELS parameter is set
+  // AUTO: row blocks from 2^12 columns and 2^19 nonzeros up, unless a
+  // PANELS parameter is set.  Blocks against panels: 200k x 200k, 2M
+  // nonzeros 20.2 -> 15.6 us; 10k x 1M, 5M 36.3 -> 26.2; 200k x 5k, 3M 19.0
+  // -> 14.5; 30k x 30k, 600k 13.0 -> 11.3; narrow uniform matrices (40 or
+  // 1,000 columns) and products under 2^19 nonzeros tie or favour panels.
   if (layout == LOOPS_PLAN_LAYOUT_AUTO)
-    layout = a.cols >= (1 << 16) && a.rows >= kBlkRows && p->panels == 0 &&
+    layout = a.cols >= (1 << 12) && a.nnz >= (1 << 19) && p->panels == 0 &&
                      p->threads_per_block == 0 && p->items_per_thread == 0
                  ? LOOPS_PLAN_LAYOUT_BLOCKS
                  : LOOPS_PLAN_LAYOUT_PANELS;

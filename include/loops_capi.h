@@ -187,7 +187,7 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
 typedef struct loops_spmv_plan* loops_spmv_plan_t;
 
 typedef enum loops_plan_layout_t {
-  LOOPS_PLAN_LAYOUT_AUTO = 0,   /* BLOCKS when cols >= 2^16, rows >= 15872
+  LOOPS_PLAN_LAYOUT_AUTO = 0,   /* BLOCKS when cols >= 2^12, nnzs >= 2^19
                                    and no PANELS parameter (panels,
                                    threads_per_block, items_per_thread) is
                                    set, else PANELS                          */

@@ -111,7 +111,7 @@ def test_plan_c2_full_size_auto_panels():
 
 
 def test_plan_auto_layout():
-    # AUTO: row blocks from 2^16 columns and 15872 rows, unless a panel
+    # AUTO: row blocks from 2^12 columns and 2^19 nonzeros, unless a panel
     # parameter is set; smaller matrices take column panels
     B = synth.powerlaw_csr(60000, 80000, 1.8, 1, 20000, 600000, seed=9, force_first_max=True)
     A = loops.csr_t.from_host(B.rows, B.cols, B.offsets, B.indices, B.values)
@@ -120,7 +120,7 @@ def test_plan_auto_layout():
         plan = loops.spmv_plan(A, **kw)
         assert plan.layout == want, kw
         plan.destroy()
-    off, idx, val, cols = MATS["skewed"]  # 3000 x 2500
+    off, idx, val, cols = MATS["skewed"]  # 3000 x 2500, under 2^12 columns
     A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
     plan = loops.spmv_plan(A)
     assert plan.layout == "panels"

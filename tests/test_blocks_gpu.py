@@ -220,3 +220,25 @@ def test_blocks_ragged_rows_and_unaligned_y(shift):
     _check(y.cpu().numpy(), off, idx, val, x_np)
     assert torch.isnan(buf[:shift]).all() and torch.isnan(buf[shift + A.rows:]).all()
     plan.destroy()
+
+
+@pytest.mark.parametrize("rows,cols,nnz,zipf", [
+    (2_000, 2_000_000, 600_000, 1.8),     # few rows, very wide: rows cut into pieces
+    (10_000, 1_000_000, 800_000, 1.8),
+    (200_000, 5_000, 700_000, 1.8),       # tall and narrow: one segment of columns in use
+    (300_000, 40, 600_000, 0.0),          # 40 columns, uniform rows of 2
+    (150_000, 150_000, 900_000, 2.2),
+])
+def test_blocks_shapes_of_the_auto_rule(rows, cols, nnz, zipf):
+    # shapes the AUTO layout sends to row blocks (>= 2^12 columns, >= 2^19
+    # nonzeros) or near its edges, built explicitly as blocks
+    lmax = min(rows // 4, cols, 50_000) if zipf else max(1, nnz // rows)
+    B = synth.powerlaw_csr(rows, cols, zipf, 1, lmax, nnz, seed=rows % 97)
+    A = loops.csr_t.from_host(B.rows, B.cols, B.offsets, B.indices, B.values)
+    plan = loops.spmv_plan(A, layout="blocks")
+    x_np = np.random.default_rng(31).uniform(-1, 1, cols).astype(np.float32)
+    y = torch.full((B.rows,), float("nan"), device="cuda")
+    plan.execute(torch.from_numpy(x_np).cuda(), y)
+    torch.cuda.synchronize()
+    _check(y.cpu().numpy(), B.offsets, B.indices, B.values, x_np)
+    plan.destroy()

@@ -228,6 +228,7 @@ def test_blocks_ragged_rows_and_unaligned_y(shift):
     (200_000, 5_000, 700_000, 1.8),       # tall and narrow: one segment of columns in use
     (300_000, 40, 600_000, 0.0),          # 40 columns, uniform rows of 2
     (150_000, 150_000, 900_000, 2.2),
+    (3_000, 20_000_000, 600_000, 1.8),    # 24 column segments of 2^20 columns
 ])
 def test_blocks_shapes_of_the_auto_rule(rows, cols, nnz, zipf):
     # shapes the AUTO layout sends to row blocks (>= 2^12 columns, >= 2^19

@@ -95,11 +95,15 @@ def spmv_f32(offsets, indices, values, x):
     return y
 
 
-def spmv_f32_mt(offsets, indices, values, x, nthreads):
+def spmv_f32_mt(offsets, indices, values, x, nthreads, out=None):
+    """Row-partitioned threads over the validator loop (each row in CSR
+    order); ``out`` (float32, rows) receives y when given."""
     offsets, indices = _c(offsets, np.int32), _c(indices, np.int32)
     values, x = _c(values, np.float32), _c(x, np.float32)
     rows = offsets.shape[0] - 1
-    y = np.empty(rows, np.float32)
+    y = np.empty(rows, np.float32) if out is None else out
+    if y.dtype != np.float32 or y.shape != (rows,) or not y.flags["C_CONTIGUOUS"]:
+        raise ValueError("out must be a contiguous float32 array of rows entries")
     if lib().oracle_spmv_f32_mt(int(nthreads), rows, offsets, indices, values, x, y) != 0:
         raise RuntimeError("oracle_spmv_f32_mt: thread creation failed")
     return y

@@ -15,7 +15,12 @@
  * Pinning (what checks each function; the tests live in tests/test_oracle.py):
  *   oracle_spmv_f32        sample.hxx 4x4 known answer, chesapeake degrees
  *                          (x = 1), FP64 bound, a planted dropped-term fails.
- *   oracle_spmv_f64 / abs  closed forms on the fixtures, |y64| <= abs.
+ *   oracle_spmv_f64 / abs  known answers an fp32 accumulator or an fp32
+ *                          product fails (2^25 + 1 - 2^25 = 1; (1+2^-23)^2
+ *                          exact in double), |y64| <= abs.
+ *   oracle_spmv_f32_mt     bit-identical to oracle_spmv_f32 for 1..16
+ *                          threads on skewed and uniform matrices (rows
+ *                          left unwritten would keep a NaN sentinel).
  *   oracle_mp_search       brute-force merge walk for every diagonal of
  *                          random tiny matrices, invariants x+y=d,
  *                          offsets[x] <= y <= offsets[x+1].
@@ -24,8 +29,12 @@
  *   oracle_wo_coords       == search at clamped tid*wpt, brute force.
  *   oracle_gm_atom_row     brute force over the exclusive scan.
  *   oracle_spmm_f32        column-by-column equality with oracle_spmv_f32.
+ *   oracle_spmm_f64        the same fp64 known answers per column, and
+ *                          column-by-column equality with oracle_spmv_f64.
+ *   oracle_count_*_errors  hand-made vectors on both sides of the bound.
  *   oracle_mtx_read        chesapeake: 39 x 39, 340 nnz (README.md:86),
- *                          symmetric, sorted (row, col).
+ *                          symmetric, sorted (row, col); duplicate entries
+ *                          keep their input order (a written file).
  */
 #include <math.h>
 #include <pthread.h>
@@ -362,13 +371,18 @@ void oracle_spmm_f64(int64_t rows, const int32_t* offsets,
 /* sizes, then with arrays of those sizes.  Returns 0 on success, negative  */
 /* codes for the reference's error conditions.                              */
 /* ------------------------------------------------------------------------ */
-typedef struct { int64_t r, c; float v; } coo_entry_t;
+typedef struct { int64_t r, c, pos; float v; } coo_entry_t;
 
+/* (row, col) order, ties (duplicate entries) kept in input order: a stable
+ * sort, as thrust 1.17's sort_by_key gives for the zip-iterator keys of
+ * coo.hxx:163-165 (non-primitive keys take its merge sort).  The position
+ * tie-break makes qsort stable. */
 static int coo_cmp(const void* a, const void* b) {
   const coo_entry_t* p = (const coo_entry_t*)a;
   const coo_entry_t* q = (const coo_entry_t*)b;
   if (p->r != q->r) return p->r < q->r ? -1 : 1;
   if (p->c != q->c) return p->c < q->c ? -1 : 1;
+  if (p->pos != q->pos) return p->pos < q->pos ? -1 : 1;
   return 0;
 }
 
@@ -398,9 +412,9 @@ int oracle_mtx_read(const char* path, int64_t* rows, int64_t* cols,
                       : fscanf(f, " %lld %lld %lf", &r, &c, &w);
     if (got != (pattern ? 2 : 3)) { free(ent); fclose(f); return -5; }
     if (r == 0 || c == 0) { free(ent); fclose(f); return -6; }
-    ent[k].r = r - 1; ent[k].c = c - 1; ent[k].v = (float)w; ++k;
+    ent[k].r = r - 1; ent[k].c = c - 1; ent[k].v = (float)w; ent[k].pos = k; ++k;
     if (symmetric && r != c) {
-      ent[k].r = c - 1; ent[k].c = r - 1; ent[k].v = (float)w; ++k;
+      ent[k].r = c - 1; ent[k].c = r - 1; ent[k].v = (float)w; ent[k].pos = k; ++k;
     }
   }
   fclose(f);

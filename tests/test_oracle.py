@@ -247,3 +247,132 @@ def test_planted_transposed_operand_is_caught(golden_dir):
             y[idx[nz]] += val[nz] * x[r]
     assert not np.array_equal(y, g["y"])
     assert deg.shape == (39,)
+
+
+# ---------------------------------------------------------------------------
+# The all-cores CPU baseline must compute what the sequential validator does
+# ---------------------------------------------------------------------------
+def _skewed_csr():
+    return random_csr(3000, 2500, 40, seed=41, long_rows=((0, 2400), (1500, 2000), (2999, 1800)))
+
+
+@pytest.mark.parametrize("nthreads", [1, 2, 7, 16])
+def test_spmv_f32_mt_bit_identical(golden_dir, nthreads):
+    from edge_loops_amd import synth
+    A = synth.c1_uniform_4k()
+    cases = [(A.offsets, A.indices, A.values, synth.uniform(A.cols, seed=11))]
+    off, idx, val = _skewed_csr()
+    cases.append((off, idx, val, np.random.default_rng(42).uniform(-1, 1, 2500).astype(np.float32)))
+    # more threads than rows, and a matrix of empty rows only
+    cases.append((np.array([0, 3, 3, 5], np.int32), np.array([0, 1, 2, 0, 2], np.int32),
+                  np.array([1, 2, 3, 4, 5], np.float32), np.array([1, 10, 100], np.float32)))
+    cases.append((np.zeros(9, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32),
+                  np.ones(4, np.float32)))
+    for off, idx, val, x in cases:
+        ref = oracle.spmv_f32(off, idx, val, x)
+        out = np.full(off.shape[0] - 1, np.nan, np.float32)  # a row left unwritten stays NaN
+        y = oracle.spmv_f32_mt(off, idx, val, x, nthreads, out=out)
+        assert y is out
+        assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+
+
+def test_planted_mt_split_mistake_is_caught():
+    # Mutant of the nnz-balanced split: each thread stops one row short of
+    # its boundary.  The NaN sentinel of the bit-identity check catches it.
+    off, idx, val = _skewed_csr()
+    x = np.random.default_rng(42).uniform(-1, 1, 2500).astype(np.float32)
+    ref = oracle.spmv_f32(off, idx, val, x)
+    rows, nnz, nt = off.shape[0] - 1, int(off[-1]), 7
+    y = np.full(rows, np.nan, np.float32)
+    r = 0
+    for t in range(nt):
+        r1 = rows if t == nt - 1 else int(np.searchsorted(off[:rows], nnz * (t + 1) // nt, "left"))
+        y[r:max(r, r1 - 1)] = ref[r:max(r, r1 - 1)]
+        r = r1
+    assert not np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# FP64 oracle: known answers that a float accumulator or product fails
+# ---------------------------------------------------------------------------
+def _cancel_case():
+    # Row 0: 2^25 + 1 - 2^25 with x = 1.  Exact: 1.  In fp32 2^25 + 1 rounds
+    # to 2^25 (its ulp is 4), so an fp32 accumulator gives 0.
+    # Row 1: (1 + 2^-23) * (1 + 2^-23) = 1 + 2^-22 + 2^-46, exact in double,
+    # 1 + 2^-22 when the product is rounded to fp32 first.
+    # Row 2: empty.  Row 3: 2^25 - 2^25 + 0.5 * 3 = 1.5.
+    e = np.float32(1 + 2.0 ** -23)
+    off = np.array([0, 3, 4, 4, 7], np.int32)
+    idx = np.array([0, 1, 2, 3, 0, 2, 4], np.int32)
+    val = np.array([2.0 ** 25, 1.0, -(2.0 ** 25), e, 2.0 ** 25, -(2.0 ** 25), 0.5], np.float32)
+    x = np.array([1, 1, 1, e, 3], np.float32)
+    y_exact = np.array([1.0, (1 + 2.0 ** -23) ** 2, 0.0, 1.5])
+    abs_exact = np.array([2.0 ** 26 + 1, (1 + 2.0 ** -23) ** 2, 0.0, 2.0 ** 26 + 1.5])
+    return off, idx, val, x, y_exact, abs_exact
+
+
+def test_spmv_f64_known_answers():
+    off, idx, val, x, y_exact, abs_exact = _cancel_case()
+    y64, a = oracle.spmv_f64(off, idx, val, x)
+    assert np.array_equal(y64, y_exact)
+    assert np.array_equal(a, abs_exact)
+    # the fp32 validator loses both: the cancellation and the product's low bits
+    y32 = oracle.spmv_f32(off, idx, val, x)
+    assert y32[0] == 0.0 and y32[1] == np.float32(1 + 2.0 ** -22)
+
+
+def test_spmm_f64_known_answers():
+    off, idx, val, x, y_exact, abs_exact = _cancel_case()
+    B = np.stack([x, 2 * x, -x], axis=1).astype(np.float32)  # scaling by 2 is exact
+    C64, A64 = oracle.spmm_f64(off, idx, val, B)
+    assert np.array_equal(C64, np.stack([y_exact, 2 * y_exact, -y_exact], axis=1))
+    assert np.array_equal(A64, np.stack([abs_exact, 2 * abs_exact, abs_exact], axis=1))
+    # column-by-column equal to the SpMV FP64 oracle on random data
+    off, idx, val = random_csr(80, 60, 30, seed=43)
+    B = np.random.default_rng(44).uniform(-1, 1, (60, 5)).astype(np.float32)
+    C64, A64 = oracle.spmm_f64(off, idx, val, B)
+    for c in range(5):
+        y64, a = oracle.spmv_f64(off, idx, val, np.ascontiguousarray(B[:, c]))
+        assert np.array_equal(C64[:, c], y64) and np.array_equal(A64[:, c], a)
+
+
+def test_planted_float_accumulator_is_caught():
+    # Mutant FP64 oracle accumulating in float32: the known answer fails.
+    off, idx, val, x, y_exact, _ = _cancel_case()
+    y = np.zeros(4)
+    for r in range(4):
+        s = np.float32(0)
+        for nz in range(off[r], off[r + 1]):
+            s = np.float32(s + np.float64(val[nz]) * np.float64(x[idx[nz]]))
+        y[r] = s
+    assert not np.array_equal(y, y_exact)
+
+
+# ---------------------------------------------------------------------------
+# Matrix Market reader: duplicates stay in input order (stable sort)
+# ---------------------------------------------------------------------------
+def test_mtx_duplicates_keep_input_order(tmp_path):
+    # Entries listed out of order with three copies of (2, 1) and two of
+    # (1, 3); each copy has its own value so the CSR order is visible.
+    p = tmp_path / "dups.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real general\n% comment\n"
+                 "3 3 8\n"
+                 "2 1 5.0\n1 3 7.0\n2 1 -1.0\n3 3 2.0\n1 3 0.25\n2 1 9.0\n1 1 4.0\n2 2 3.0\n")
+    rows, cols, off, idx, val = oracle.mtx_read(str(p))
+    assert (rows, cols) == (3, 3)
+    assert off.tolist() == [0, 3, 7, 8]
+    assert idx.tolist() == [0, 2, 2, 0, 0, 0, 1, 2]
+    assert val.tolist() == [4.0, 7.0, 0.25, 5.0, -1.0, 9.0, 3.0, 2.0]
+
+
+def test_mtx_symmetric_expansion_order(tmp_path):
+    # symmetric: (r, c) then its mirror (c, r), in entry order; a diagonal
+    # entry is not mirrored (market.hxx:187-223).  Duplicate mirrors keep
+    # the order in which they were produced.
+    p = tmp_path / "sym.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate integer symmetric\n"
+                 "3 3 4\n2 1 1\n3 1 2\n2 1 3\n3 3 4\n")
+    rows, cols, off, idx, val = oracle.mtx_read(str(p))
+    assert off.tolist() == [0, 3, 5, 7]
+    assert idx.tolist() == [1, 1, 2, 0, 0, 0, 2]
+    assert val.tolist() == [1.0, 3.0, 2.0, 1.0, 3.0, 2.0, 4.0]

@@ -206,7 +206,7 @@ SIGNATURES = {
     "loops_server_destroy": (_st, [_vp]),
     "loops_server_add_matrix": (_st, [_vp, ctypes.POINTER(ServerMatrix),
                                       ctypes.POINTER(ctypes.c_int32)]),
-    "loops_server_submit": (_st, [_vp, ctypes.c_int32, _i64, _vp, _i64,
+    "loops_server_submit": (_st, [_vp, ctypes.c_int32, _i64, _vp, _i64, _vp,
                                   ctypes.POINTER(ctypes.c_int64)]),
     "loops_server_test": (_st, [_vp, _i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_vp),
                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),

@@ -18,6 +18,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _capi
+from . import _validate as _v
 from .hip import Event, stream_synchronize, timer_t
 
 
@@ -65,6 +66,11 @@ def spmv_workspace_size(schedule, csr, **tuning):
 def run_spmv(schedule, csr, x, y, stream=None, workspace=None, **tuning):
     """Asynchronous y = A x through loops_spmv (no synchronisation)."""
     sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
+    _v.check_csr(csr)
+    _v.check_vector(x, csr.cols, "x")
+    _v.check_vector(y, csr.rows, "y")
+    _v.check_workspace(workspace)
+    _v.check_devices(csr, x=x, y=y, workspace=workspace)
     v = csr.view()
     p = make_params(**tuning)
     ws = _ptr(workspace) if workspace is not None else None
@@ -88,7 +94,14 @@ def spmm_workspace_size(schedule, csr, k, **tuning):
 def run_spmm(schedule, csr, B, C, stream=None, workspace=None, **tuning):
     """Asynchronous C = A B (row-major fp32 B: cols x k, C: rows x k)."""
     sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
-    k = B.shape[1]
+    _v.check_csr(csr)
+    if B.dim() != 2:
+        raise ValueError("B must be 2-D (cols x k)")
+    k = int(B.shape[1])
+    _v.check_dense(B, csr.cols, k, "B")
+    _v.check_dense(C, csr.rows, k, "C")
+    _v.check_workspace(workspace)
+    _v.check_devices(csr, B=B, C=C, workspace=workspace)
     v = csr.view()
     p = make_params(**tuning)
     ws = _ptr(workspace) if workspace is not None else None
@@ -175,6 +188,8 @@ class spmv_plan:
 
     def __init__(self, csr, panels=0, threads_per_block=0, items_per_thread=0,
                  stream=None, layout="auto"):
+        _v.check_csr(csr)
+        _v.check_devices(csr)
         self.csr = csr  # keeps A alive: with one panel the plan reads it in place
         self.rows, self.cols = csr.rows, csr.cols
         v = csr.view()
@@ -201,6 +216,12 @@ class spmv_plan:
         self.blocks, self.compressed, self.split_rows = int(nb.value), bool(comp.value), int(ns.value)
 
     def execute(self, x, y, stream=None, start_event=None, stop_event=None):
+        """y = A x (asynchronous).  Executions of one plan must not overlap
+        in time (another stream, or graph replays on several streams): they
+        share the plan's partial sums."""
+        _v.check_vector(x, self.cols, "x")
+        _v.check_vector(y, self.rows, "y")
+        _v.check_devices(self.csr, x=x, y=y)
         _capi.check(_capi.lib().loops_spmv_plan_execute(
             self._h, _ptr(x), _ptr(y),
             start_event.ptr if start_event is not None else None,
@@ -224,6 +245,8 @@ def merge_path_partition(csr, items_per_tile, stream=None):
     the reference's generate_search_coordinates
     (schedule/merge_path_flat.hxx:108-138)."""
     import torch
+    _v.check_csr(csr)
+    _v.check_devices(csr)
     T = ctypes.c_int64()
     _capi.check(_capi.lib().loops_merge_path_num_tiles(
         csr.rows, csr.nnzs, int(items_per_tile), ctypes.byref(T)),
@@ -239,6 +262,8 @@ def work_oriented_partition(csr, num_threads, stream=None):
     """Per-thread (start, end) coordinates, shape (num_threads, 4)
     (schedule/work_oriented.hxx:150-167)."""
     import torch
+    _v.check_csr(csr)
+    _v.check_devices(csr)
     coords = torch.empty((int(num_threads), 4), dtype=torch.int32, device=csr.offsets.device)
     _capi.check(_capi.lib().loops_work_oriented_partition(
         _ptr(csr.offsets), csr.rows, csr.nnzs, int(num_threads), _ptr(coords),

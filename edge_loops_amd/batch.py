@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _capi
+from . import _validate as _v
 
 
 class BatchItem:
@@ -23,6 +24,9 @@ class BatchItem:
         self.items_per_thread = int(items_per_thread)
 
     def to_c(self):
+        _v.check_csr(self.csr)
+        _v.check_out(self.csr, self.y, _v.check_rhs(self.csr, self.x, "x"), "y")
+        _v.check_devices(self.csr, x=self.x, y=self.y)
         k = 1 if self.x.dim() == 1 else int(self.x.shape[1])
         ldx = 1 if self.x.dim() == 1 else int(self.x.stride(0))
         ldy = 1 if self.y.dim() == 1 else int(self.y.stride(0))

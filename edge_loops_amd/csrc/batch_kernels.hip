@@ -32,6 +32,8 @@
 // the batch.  Launching allocates nothing and can be captured in a hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cstdint>
 #include <cstring>
 #include <type_traits>
@@ -729,18 +731,21 @@ struct spmm_kernel_table {
 const spmm_kernel_table g_spmm_kernels;
 
 // Resident workgroups of each persistent kernel on a device (occupancy x CUs),
-// computed once per device.
+// computed once per device under a lock (the server thread and its clients
+// may ask at the same time).
 struct batch_grids {
   int spmv4 = 0, spmv8 = 0, spmv16 = 0;
   int spmm[kSpmmVariants] = {};
 };
 constexpr int kMaxDevices = 64;
 batch_grids g_grids[kMaxDevices];
+std::mutex g_grids_mu;
 
 hipError_t resident_grids(batch_grids* out) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_grids_mu);
   if (dev >= 0 && dev < kMaxDevices && g_grids[dev].spmv4 > 0) {
     *out = g_grids[dev];
     return hipSuccess;

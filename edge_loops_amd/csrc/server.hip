@@ -99,6 +99,7 @@ struct request_t {
   int64_t k = 1;
   const float* x = nullptr;
   int64_t ldx = 1;
+  hipEvent_t ready = nullptr;  // the producer of x (caller's event), or null
   loops_schedule_t schedule = LOOPS_SCHEDULE_MERGE_PATH_FLAT;
   int32_t ipt = 0;
   size_t y_off = 0, y_bytes = 0;
@@ -244,8 +245,12 @@ loops_status_t one_workspace(loops_server* S, const request_t* r, size_t* need) 
                                    need);
 }
 
-// Launches one batch (or, in plain mode, one request) on slot sl.
+// Launches one batch (or, in plain mode, one request) on slot sl, after the
+// producers of the requests' inputs (their ready events).
 loops_status_t launch(loops_server* S, slot_t& sl, const std::vector<request_t*>& batch) {
+  for (const request_t* r : batch)
+    if (r->ready && hipStreamWaitEvent(sl.stream, r->ready, 0) != hipSuccess)
+      return LOOPS_ERROR_HIP;
   if (S->plain()) {
     const loops_status_t st =
         launch_one(S, batch[0], S->plain_ws, (size_t)S->cfg.scratch_bytes, sl.stream);
@@ -510,7 +515,7 @@ loops_status_t loops_server_add_matrix(loops_server_t S, const loops_server_matr
 }
 
 loops_status_t loops_server_submit(loops_server_t S, int32_t matrix, int64_t k, const float* x,
-                                   int64_t ldx, int64_t* ticket) {
+                                   int64_t ldx, hipEvent_t ready, int64_t* ticket) {
   if (!S || !ticket || k < 1) return LOOPS_ERROR_INVALID_VALUE;
   request_t* r = new (std::nothrow) request_t();
   if (!r) return LOOPS_ERROR_ALLOCATION;
@@ -529,6 +534,7 @@ loops_status_t loops_server_submit(loops_server_t S, int32_t matrix, int64_t k, 
   r->k = k;
   r->x = x;
   r->ldx = k > 1 ? ldx : 1;
+  r->ready = ready;
   pick(r->m, k, &r->schedule, &r->ipt);
   r->y_bytes = (size_t)r->m.A.rows * (size_t)k * sizeof(float);
   // Bounds of this request's share of a batch: the sizes of a batch of it alone.
@@ -659,7 +665,7 @@ loops_status_t loops_server_replay(loops_server_t S, const loops_server_request_
     while (err == LOOPS_SUCCESS && next < n && arrival_s[next] <= t) {
       int64_t tk = 0;
       const loops_server_request_t& q = reqs[next];
-      err = loops_server_submit(S, q.matrix, q.k, q.x, q.ldx, &tk);
+      err = loops_server_submit(S, q.matrix, q.k, q.x, q.ldx, nullptr, &tk);
       if (err != LOOPS_SUCCESS) break;
       open.emplace_back(next, tk);
       ++next;

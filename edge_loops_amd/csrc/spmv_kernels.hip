@@ -204,16 +204,26 @@ __global__ __launch_bounds__(TPB) void spmv_group_mapped_kernel(
   s_acc[threadIdx.x] = 0.f;
   p.sync();
 
-  for (int base = 0; base < aggregate; base += TPT) {
+  // Groups of several wavefronts scan across the whole block (workgroup
+  // barriers inside), so every group runs the block's largest trip count;
+  // keys are then block-wide (group base + group-local row) and a lane past
+  // its group's nonzeros takes a sentinel of its own group.
+  int trip = aggregate;
+  if constexpr (TPT > 64 && TPT < TPB) {
+    for (int g = 0; g < TPB / TPT; ++g)
+      trip = storage.tile_aggregates[g] > trip ? storage.tile_aggregates[g] : trip;
+  }
+  constexpr int kSentinel = TPT > 64 ? TPB : TPT;
+  for (int base = 0; base < trip; base += TPT) {
     int v_atom = base + rank;
-    int key = TPT;  // sentinel for lanes past the group's nonzeros
+    int key = kSentinel + (TPT > 64 ? (int)p.meta_group_rank() : 0);
     float prod = 0.f;
     if (v_atom < aggregate) {
       int v_tile = config.tile_accessor(v_atom, p);
       int row = config.tile_id(v_tile, p);
       int nz = config.atom_id(v_atom, row, v_tile, p);
       prod = value_traits<V>::load1(values + nz) * x[ld_idx(indices + nz)];
-      key = v_tile;
+      key = TPT > 64 ? gbase + v_tile : v_tile;
     }
     float v = prod;
     int head, tail;
@@ -224,6 +234,7 @@ __global__ __launch_bounds__(TPB) void spmv_group_mapped_kernel(
       tail = (rank == TPT - 1 || kn != key) ? 1 : 0;
       int f = head;
       group_seg_scan<TPT>(v, f);
+      if (tail && key < TPT) s_acc[gbase + key] = s_acc[gbase + key] + v;
     } else {
       s_key[threadIdx.x] = key;
       __syncthreads();
@@ -233,8 +244,8 @@ __global__ __launch_bounds__(TPB) void spmv_group_mapped_kernel(
       float pv;
       int pf;
       block_seg_scan<TPB>(v, f, s_wv, s_wf, pv, pf);
+      if (tail && key < TPB) s_acc[key] = s_acc[key] + v;
     }
-    if (tail && key < TPT) s_acc[gbase + key] = s_acc[gbase + key] + v;
     p.sync();
   }
   if (rank < config.get_length(p)) {
@@ -485,6 +496,7 @@ hipError_t gm_dispatch_v(const csr_args& A, const float* x, float* y,
   X(64, 64)                 \
   X(128, 128)               \
   X(256, 256)               \
+  X(256, 128)               \
   X(128, 64)                \
   X(256, 64)                \
   X(256, 32)                \

@@ -7,7 +7,10 @@
 //   thread_mapped   setup<thread_mapped>: tiles() / atoms(row)
 //   block_mapped    setup<group_mapped, 128, 128>: partition / atom_accessor /
 //                   tile_accessor / tile_id / atom_id, atomicAdd per nonzero
-//   warp_mapped     the same with 64-lane groups in 256-lane blocks
+//   warp_mapped     the same with 32-lane groups in 256-lane blocks (the
+//                   reference's warp_mapped, group_mapped.hxx:239-241)
+//   wave_mapped     64-lane groups (one wavefront) in 256-lane blocks
+//   group128        128-lane groups (two wavefronts) in 256-lane blocks
 //   work_oriented   setup<work_oriented>: init / tiles / atoms / remainder_*,
 //                   every partial row added with atomicAdd (the reference
 //                   stores complete rows, which races with another block's
@@ -154,10 +157,16 @@ int main(int argc, char** argv) {
   launch::non_cooperative(0, k_group_mapped<128, 128>, dim3(grid128), dim3(128), R, N, off,
                           idx, val, x.raw(), y.raw());
   save("block_mapped");
-  launch::non_cooperative(0, k_group_mapped<256, 64>,
-                          dim3((unsigned)math::ceil_div(R, std::size_t(256))), dim3(256),
-                          R, N, off, idx, val, x.raw(), y.raw());
+  const unsigned grid256 = (unsigned)math::ceil_div(R, std::size_t(256));
+  launch::non_cooperative(0, k_group_mapped<256, 32>, dim3(grid256), dim3(256), R, N, off,
+                          idx, val, x.raw(), y.raw());
   save("warp_mapped");
+  launch::non_cooperative(0, k_group_mapped<256, 64>, dim3(grid256), dim3(256), R, N, off,
+                          idx, val, x.raw(), y.raw());
+  save("wave_mapped");
+  launch::non_cooperative(0, k_group_mapped<256, 128>, dim3(grid256), dim3(256), R, N, off,
+                          idx, val, x.raw(), y.raw());
+  save("group128");
   {
     device::properties_t props;
     launch::non_cooperative(0, k_work_oriented<128>,

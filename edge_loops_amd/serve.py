@@ -19,6 +19,7 @@ import ctypes
 import numpy as np
 
 from . import _capi, hip, tune
+from . import _validate as _v
 
 MODES = {"batched": _capi.SERVER_BATCHED, "plain": _capi.SERVER_PLAIN}
 
@@ -60,6 +61,8 @@ class Server:
         self.inputs = {}     # ticket -> x (alive until the request completes)
 
     def add_matrix(self, csr, table=None, max_row=None, tunings=None):
+        _v.check_csr(csr)
+        _v.check_devices(csr)
         m = tunings if tunings is not None else matrix_tunings(csr, table, max_row)
         mid = ctypes.c_int32()
         _capi.check(_capi.lib().loops_server_add_matrix(self.ptr, ctypes.byref(m),
@@ -68,14 +71,25 @@ class Server:
         self.matrices.append((csr, m))
         return mid.value
 
-    def submit(self, matrix, x):
+    def submit(self, matrix, x, stream=None):
+        """Queue Y = A x.  x (fp32 device vector, or cols x k with unit
+        column stride) may still be in production on `stream` (default:
+        torch's current stream): an event recorded there now orders the
+        server's read of x after it."""
+        import torch
+        if 0 <= matrix < len(self.matrices):  # an unknown id is the library's to reject
+            _v.check_rhs(self.matrices[matrix][0], x)
+            _v.check_devices(self.matrices[matrix][0], x=x)
         k = 1 if x.dim() == 1 else int(x.shape[1])
         ldx = 1 if x.dim() == 1 else int(x.stride(0))
+        s = torch.cuda.current_stream(x.device) if stream is None else stream
+        ready = hip.Event()
+        ready.record(s.cuda_stream if hasattr(s, "cuda_stream") else s)
         t = ctypes.c_int64()
         _capi.check(_capi.lib().loops_server_submit(self.ptr, int(matrix), k, x.data_ptr(),
-                                                    ldx, ctypes.byref(t)),
+                                                    ldx, ready.ptr, ctypes.byref(t)),
                     "loops_server_submit")
-        self.inputs[t.value] = (matrix, k, x)
+        self.inputs[t.value] = (matrix, k, x, ready)
         return t.value
 
     def wait(self, ticket):
@@ -95,7 +109,7 @@ class Server:
     def result(self, ticket):
         """Wait for a request, copy its output to a new device tensor, release."""
         import torch
-        matrix, k, _ = self.inputs[ticket]
+        matrix, k = self.inputs[ticket][:2]
         rows = self.matrices[matrix][0].rows
         yp, _, _ = self.wait(ticket)
         out = torch.empty((rows,) if k == 1 else (rows, k), device="cuda")
@@ -107,8 +121,14 @@ class Server:
     def replay(self, requests, arrivals):
         """Open-loop load: requests = [(matrix, x)], arrivals in seconds
         (non-decreasing).  Returns (latencies in s, makespan in s)."""
+        import torch
         n = len(requests)
         arr = (_capi.ServerRequest * max(n, 1))()
+        for m, x in requests:
+            if 0 <= m < len(self.matrices):
+                _v.check_rhs(self.matrices[m][0], x)
+                _v.check_devices(self.matrices[m][0], x=x)
+        torch.cuda.synchronize()  # the replayed inputs are complete on the device
         for i, (m, x) in enumerate(requests):
             k = 1 if x.dim() == 1 else int(x.shape[1])
             arr[i] = _capi.ServerRequest(int(m), k, x.data_ptr(),

@@ -3,7 +3,8 @@
  * @brief Range-for proxies returned by the schedules' tiles()/atoms().
  *
  * Same interface as the reference's include/loops/range.hxx:53-216:
- * range(b, e), range(b) (unbounded), .step(s), indices(container).
+ * range(b, e), range(b) (unbounded), .step(s), indices(container | array |
+ * initializer list).
  * Termination of a stepped range is "current passed end" (range.hxx:78-80);
  * the unit-step range here also stops once current >= end, so a range whose
  * begin exceeds its end is empty instead of running forever.
@@ -13,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <initializer_list>
 #include <type_traits>
 
 namespace loops {
@@ -127,6 +129,13 @@ __host__ __device__ auto indices(C const& cont)
 template <typename type_t, std::size_t N>
 __host__ __device__ range_proxy<std::size_t> indices(type_t (&)[N]) {
   return {0, N};
+}
+
+/// indices({a, b, c}): 0 .. list size (range.hxx:220-224).
+template <typename type_t>
+__host__ __device__ range_proxy<typename std::initializer_list<type_t>::size_type>
+indices(std::initializer_list<type_t>&& cont) {
+  return {0, cont.size()};
 }
 
 }  // namespace loops

@@ -5,8 +5,9 @@
  * atoms_t, tile_size_t, atom_size_t> (reference include/loops/schedule.hxx:
  * 23-63).  "Tiles" are rows, "atoms" are nonzeros.
  *
- * Written for CDNA: wavefronts are 64 lanes, so the warp-sized group
- * (warp_mapped) is 64 lanes and group scans use wave shuffles within 64.
+ * Written for CDNA: wavefronts are 64 lanes.  warp_mapped keeps the
+ * reference's 32-lane groups; wave_mapped is the 64-lane wavefront; group
+ * scans use wave shuffles within 64 lanes.
  */
 #pragma once
 

@@ -6,10 +6,14 @@
  * 80-248).
  *
  * CDNA notes: groups of up to 64 lanes live inside one wavefront and scan
- * with wave shuffles; a block-sized group (block_mapped) scans through LDS.
- * Group sizes between 64 and the block size are not supported (a group
- * barrier inside a data-dependent loop would need every group of the block
- * to agree on trip counts).  warp_mapped is the 64-lane wavefront.
+ * with wave shuffles.  Groups of several wavefronts (a multiple of 64 lanes
+ * dividing the block, block_mapped included) scan each wavefront with
+ * shuffles and carry the wavefront totals through LDS; their sync() is a
+ * workgroup barrier, so, as with the reference's block-sized groups, every
+ * lane of the block must reach it (atom_accessor is called by all lanes).
+ * warp_mapped keeps the reference's 32 lanes (group_mapped.hxx:239-241), so
+ * its row-to-group partition is the reference's; wave_mapped is the 64-lane
+ * wavefront of CDNA.
  */
 #pragma once
 
@@ -24,8 +28,10 @@ namespace schedule {
 template <std::size_t block_size, std::size_t tile_size>
 struct thread_group_t {
   static_assert(tile_size == block_size ||
-                    (tile_size <= 64 && (tile_size & (tile_size - 1)) == 0),
-                "group size must be a power of two <= 64 or the block size");
+                    (tile_size <= 64 && (tile_size & (tile_size - 1)) == 0) ||
+                    (tile_size % 64 == 0 && block_size % tile_size == 0),
+                "group size must be a power of two <= 64, a multiple of 64 "
+                "dividing the block size, or the block size");
   static_assert(block_size % 64 == 0 || block_size == tile_size,
                 "block size must be whole wavefronts");
   __device__ unsigned int thread_rank() const {
@@ -57,6 +63,8 @@ class atom_traits<algorithms_t::group_mapped, atoms_type, atom_size_type> {
   __host__ __device__ atom_traits() : size_(0), atoms_(nullptr) {}
   __host__ __device__ atom_traits(atom_size_t size)
       : size_(size), atoms_(nullptr) {}
+  __host__ __device__ atom_traits(atom_size_t size, atoms_iterator_t atoms)
+      : size_(size), atoms_(atoms) {}
   __host__ __device__ atom_size_t size() const { return size_; }
   __host__ __device__ atoms_iterator_t begin() { return atoms_; }
   __host__ __device__ atoms_iterator_t end() { return atoms_ + size_; }
@@ -209,7 +217,9 @@ class setup<algorithms_t::group_mapped, THREADS_PER_BLOCK, THREADS_PER_TILE,
     return (long long)blockIdx.x * blockDim.x + threadIdx.x;
   }
 
-  /// Exclusive prefix sum over the group (integer, exact).
+  /// Exclusive prefix sum over the group (integer, exact).  A group of
+  /// several wavefronts adds the totals of its earlier wavefronts, read
+  /// from LDS between two workgroup barriers.
   __device__ atoms_t exclusive_scan(partition_t& p, atoms_t v) {
     constexpr int width = threads_per_tile < 64 ? threads_per_tile : 64;
     const int lane_in = threadIdx.x % width;
@@ -222,11 +232,13 @@ class setup<algorithms_t::group_mapped, THREADS_PER_BLOCK, THREADS_PER_TILE,
     if constexpr (threads_per_tile <= 64) {
       return inc - v;
     } else {
+      constexpr int waves_per_group = threads_per_tile / 64;
       const int wave = threadIdx.x / 64;
+      const int first_wave = wave - wave % waves_per_group;
       if ((threadIdx.x & 63) == 63) buffer.wave_totals[wave] = inc;
       __syncthreads();
       atoms_t carry = 0;
-      for (int w = 0; w < wave; ++w) carry += buffer.wave_totals[w];
+      for (int w = first_wave; w < wave; ++w) carry += buffer.wave_totals[w];
       __syncthreads();
       return carry + inc - v;
     }
@@ -235,6 +247,10 @@ class setup<algorithms_t::group_mapped, THREADS_PER_BLOCK, THREADS_PER_TILE,
 
 template <std::size_t threads_per_block, typename tiles_t, typename atoms_t>
 using warp_mapped =
+    setup<algorithms_t::group_mapped, threads_per_block, 32, tiles_t, atoms_t>;
+
+template <std::size_t threads_per_block, typename tiles_t, typename atoms_t>
+using wave_mapped =
     setup<algorithms_t::group_mapped, threads_per_block, 64, tiles_t, atoms_t>;
 
 template <std::size_t threads_per_block, typename tiles_t, typename atoms_t>

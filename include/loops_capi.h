@@ -190,7 +190,15 @@ typedef enum loops_plan_layout_t {
   LOOPS_PLAN_LAYOUT_AUTO = 0,   /* BLOCKS when cols >= 2^12, nnzs >= 2^19
                                    and no PANELS parameter (panels,
                                    threads_per_block, items_per_thread) is
-                                   set, else PANELS                          */
+                                   set, else PANELS.  The rule weighs
+                                   execution time only: a BLOCKS plan costs
+                                   1.5-45 ms to build (several stream
+                                   synchronisations) and about 1.45x the
+                                   CSR's device memory, and repays the build
+                                   against per-call loops_spmv after roughly
+                                   200 executions (against a PANELS plan:
+                                   350-900).  For a matrix used a few times
+                                   call loops_spmv or ask for PANELS.       */
   LOOPS_PLAN_LAYOUT_PANELS = 1,
   LOOPS_PLAN_LAYOUT_BLOCKS = 2
 } loops_plan_layout_t;
@@ -207,7 +215,11 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
                                       const loops_plan_params_t* params,
                                       hipStream_t stream,
                                       loops_spmv_plan_t* plan);
-/* y = A * x (y overwritten); events (optional) around the main kernel. */
+/* y = A * x (y overwritten); events (optional) around the main kernel.
+ * Executions of one plan must not overlap in time (two streams, or graph
+ * replays on several streams): they share the plan's partial sums (BLOCKS)
+ * or fix-up slots and partial y (PANELS).  Serialise them, or build one
+ * plan per concurrent user. */
 loops_status_t loops_spmv_plan_execute(loops_spmv_plan_t plan, const float* x,
                                        float* y, hipEvent_t start_event,
                                        hipEvent_t stop_event,
@@ -350,9 +362,16 @@ loops_status_t loops_server_add_matrix(loops_server_t server,
                                        int32_t* id);
 /* Request Y = A X with X (cols x k, leading dimension ldx >= k; a plain
  * vector when k == 1) in device memory, valid until the request completes.
- * The output (rows x k, leading dimension k) is allocated by the server. */
+ * The server runs on streams of its own, so it orders its read of X after
+ * the work that produced X only through `ready`: an event recorded after
+ * that work on the producer's stream (the server's stream waits for it
+ * before the request's kernels), or NULL when X is complete on the device
+ * at submit time.  The event must not be recorded again or destroyed until
+ * the request completes.  The output (rows x k, leading dimension k) is
+ * allocated by the server. */
 loops_status_t loops_server_submit(loops_server_t server, int32_t matrix, int64_t k,
-                                   const float* x, int64_t ldx, int64_t* ticket);
+                                   const float* x, int64_t ldx, hipEvent_t ready,
+                                   int64_t* ticket);
 /* done = 1 once complete; y then points at the output until release.
  * Times are seconds on the server's clock (loops_server_now). */
 loops_status_t loops_server_test(loops_server_t server, int64_t ticket, int32_t* done,
@@ -365,7 +384,8 @@ loops_status_t loops_server_now(loops_server_t server, double* now_s);
 loops_status_t loops_server_stats(loops_server_t server, loops_server_stats_t* stats);
 
 /* Open-loop load: request i is submitted arrival_s[i] seconds (non-decreasing)
- * after the call starts and released once complete; latency_s[i] =
+ * after the call starts (X complete on the device beforehand: no ready
+ * event) and released once complete; latency_s[i] =
  * completion - scheduled arrival; makespan_s = last completion - first
  * arrival. */
 typedef struct loops_server_request_t {

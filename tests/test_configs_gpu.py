@@ -1,0 +1,174 @@
+"""The batched and served configurations at their own workloads.
+
+Batched: the 512-matrix set (rows log-uniform in [1e3, 2e5], nnz/row
+Zipf(1.8) capped at 4096, fp32), one persistent launch with the autotuner's
+picks, captured in a graph and replayed; every row of every matrix within
+1e-5 of sum_j |a_ij x_j| of the FP64 oracle, each replay bit-identical to an
+eager launch, and items bit-identical to the same product launched alone.
+
+Served: the 64-matrix pool (10k-4M rows, 0.1M-8M nonzeros, Zipf 1.8-2.2)
+resident in HBM, a seeded Poisson stream of requests with 1-32 fp32
+right-hand sides; every result bit-identical alone and in the mix, sampled
+rows within the 1e-5 bar of the FP64 oracle, fp32 and bf16 storage (bf16
+against the FP64 product of the stored values).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import edge_loops_amd as loops  # noqa: E402
+from edge_loops_amd import synth, tune  # noqa: E402
+from edge_loops_amd.batch import Batch, BatchItem  # noqa: E402
+
+REL = 1e-5
+
+
+def _sub_csr(A_h, rows, values):
+    """The CSR of the given rows of A (values may be the stored ones)."""
+    lens = A_h.offsets[rows + 1] - A_h.offsets[rows]
+    off = np.zeros(rows.shape[0] + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    if rows.shape[0]:
+        idx = np.concatenate([A_h.indices[A_h.offsets[r]:A_h.offsets[r + 1]] for r in rows])
+        val = np.concatenate([values[A_h.offsets[r]:A_h.offsets[r + 1]] for r in rows])
+    else:
+        idx, val = np.zeros(0, np.int32), np.zeros(0, np.float32)
+    return off.astype(np.int32), idx, val
+
+
+# ---------------------------------------------------------------------------
+# Batched: 512 matrices
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def c4():
+    mats = synth.c4_batch_512()
+    table = tune.Table.load()
+    picks = table.pick_batch([(m.rows, m.nnz, m.max_row(), 1) for m in mats])
+    dev = [loops.csr_t.from_host(m.rows, m.cols, m.offsets, m.indices, m.values) for m in mats]
+    xs = [torch.empty(m.cols, device="cuda") for m in mats]
+    ys = [torch.full((m.rows,), float("nan"), device="cuda") for m in mats]
+    items = [BatchItem(A, x, y, s, i) for A, x, y, (s, i) in zip(dev, xs, ys, picks)]
+    return mats, picks, items
+
+
+def test_c4_batch_tuned_graph_replay(c4):
+    mats, picks, items = c4
+    assert len(mats) == 512
+    b = Batch(items)
+    g = b.capture()
+    for rep in range(2):
+        xh = [synth.uniform(m.cols, seed=1000 * rep + i) for i, m in enumerate(mats)]
+        for it, x in zip(items, xh):
+            it.x.copy_(torch.from_numpy(x))
+            it.y.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        replayed = [it.y.cpu().numpy().copy() for it in items]
+        b.launch()
+        torch.cuda.synchronize()
+        bad = 0
+        for i, (m, it, yr, x) in enumerate(zip(mats, items, replayed, xh)):
+            assert np.array_equal(yr.view(np.uint32), it.y.cpu().numpy().view(np.uint32)), i
+            y64, a = oracle.spmv_f64(m.offsets, m.indices, m.values, x)
+            n, worst = oracle.count_rel_errors(yr, y64, a, REL)
+            bad += n
+            assert n == 0, (i, picks[i], worst)
+        assert bad == 0
+
+
+def test_c4_items_alone_equal_in_the_mix(c4):
+    """A seeded sample of 24 items: the batch result equals the item alone
+    (a batch of one) and, for merge-path items, the per-call entry point
+    with the same tile (256 lanes x items_per_thread steps)."""
+    mats, picks, items = c4
+    for it, i in zip(items, range(len(items))):
+        it.x.copy_(torch.from_numpy(synth.uniform(mats[i].cols, seed=7 + i)))
+    Batch(items).launch()
+    torch.cuda.synchronize()
+    sample = np.random.default_rng(4).choice(len(items), 24, replace=False)
+    for i in sample:
+        it = items[i]
+        mixed = it.y.cpu().numpy().copy()
+        it.y.fill_(float("nan"))
+        Batch([it]).launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(it.y.cpu().numpy().view(np.uint32), mixed.view(np.uint32)), i
+        if it.schedule == "merge_path_flat":
+            y = torch.full_like(it.y, float("nan"))
+            loops.run_spmv("merge_path_flat", it.csr, it.x, y, threads_per_block=256,
+                           items_per_thread=it.items_per_thread or 8)
+            torch.cuda.synchronize()
+            assert np.array_equal(y.cpu().numpy().view(np.uint32), mixed.view(np.uint32)), i
+
+
+# ---------------------------------------------------------------------------
+# Served: 64 resident matrices, Poisson stream
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def c5():
+    mats = synth.c5_pool_64()
+    rng = np.random.default_rng(55)
+    n = 64
+    mid = rng.integers(0, len(mats), n)
+    ks = rng.integers(1, 33, n)
+    arrivals = np.cumsum(rng.exponential(1.0 / 5000.0, n))
+    xs = []
+    for j, (m, k) in enumerate(zip(mid, ks)):
+        cols = mats[m].cols
+        xs.append(synth.uniform(cols, seed=300 + j) if k == 1 else
+                  synth.dense(cols, int(k), seed=300 + j))
+    return mats, mid, ks, arrivals, xs
+
+
+@pytest.mark.parametrize("vt", ["f32", "bf16"])
+def test_c5_served_alone_and_in_the_mix(c5, vt):
+    from edge_loops_amd.serve import Server
+    mats, mid, ks, arrivals, xs = c5
+    table = tune.Table.load()
+    with Server(arena_bytes=16 << 30, scratch_bytes=1 << 30) as srv:
+        dev = [loops.csr_t.from_host(m.rows, m.cols, m.offsets, m.indices, m.values,
+                                     value_type=vt) for m in mats]
+        ids = [srv.add_matrix(A, table=table, max_row=m.max_row()) for A, m in zip(dev, mats)]
+        X = [torch.from_numpy(x).cuda() for x in xs]
+        # alone: one request at a time
+        alone = [srv.result(srv.submit(ids[m], x)).cpu().numpy() for m, x in zip(mid, X)]
+        # in the mix: the Poisson stream replayed open-loop, then every
+        # request of it submitted at once
+        lat, span = srv.replay([(ids[m], x) for m, x in zip(mid, X)], arrivals)
+        assert np.all(lat > 0) and span > 0
+        tickets = [srv.submit(ids[m], x) for m, x in zip(mid, X)]
+        mixed = [srv.result(t).cpu().numpy() for t in tickets]
+        st = srv.stats()
+        assert st["batches"] < st["completed"]
+    srng = np.random.default_rng(56)
+
+    def f64(A_h, values, rows, X):
+        off, idx, val = _sub_csr(A_h, rows, values)
+        if X.ndim == 1:
+            return oracle.spmv_f64(off, idx, val, X)
+        return oracle.spmm_f64(off, idx, val, X)
+
+    for j, (m, k) in enumerate(zip(mid, ks)):
+        A_h = mats[m]
+        values = dev[m].values.float().cpu().numpy() if vt != "f32" else A_h.values
+        diff = np.argwhere(alone[j].view(np.uint32) != mixed[j].view(np.uint32))
+        if diff.shape[0]:
+            rows = np.unique(diff[:, 0])[:64]
+            r64, a64 = f64(A_h, values, rows, xs[j])
+            err_a = float(np.nanmax(np.abs(alone[j][rows] - r64) / (a64 + 1e-30)))
+            err_m = float(np.nanmax(np.abs(mixed[j][rows] - r64) / (a64 + 1e-30)))
+            pytest.fail(f"request {j} (matrix {m}, k {k}, {A_h.rows} rows): {diff.shape[0]} "
+                        f"entries differ alone / in the mix, first {diff[:4].tolist()}, last "
+                        f"{diff[-2:].tolist()}; worst error vs FP64 there: alone {err_a:.3g}, "
+                        f"mixed {err_m:.3g}")
+        rows = np.sort(np.unique(np.concatenate([
+            srng.choice(A_h.rows, min(300, A_h.rows), replace=False),
+            [int(np.argmax(np.diff(A_h.offsets)))]])))
+        r64, a64 = f64(A_h, values, rows, xs[j])
+        got = mixed[j][rows]
+        assert np.all(np.abs(got - r64) <= REL * a64 + 1e-30), (j, m, k)

@@ -65,7 +65,8 @@ def test_schedule_api_user_kernels(skewed_mtx, tmp_path):
     y64, a = oracle.spmv_f64(off, idx, val, x)
     # User code is built like a user's build (hipcc defaults, multiply-add
     # contraction allowed), so only the 1e-5 row bar applies.
-    for name in ("thread_mapped", "block_mapped", "warp_mapped", "work_oriented", "merge_path_flat"):
+    for name in ("thread_mapped", "block_mapped", "warp_mapped", "wave_mapped", "group128",
+                 "work_oriented", "merge_path_flat"):
         assert f"{name} done" in out
         y = np.fromfile(tmp_path / f"y_{name}.bin", dtype=np.float32)
         bad, worst = oracle.count_rel_errors(y, y64, a, 1e-5)

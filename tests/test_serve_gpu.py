@@ -147,3 +147,28 @@ def test_rejects_bad_requests():
             srv.submit(mid, x)
         with pytest.raises(loops.LoopsError):
             srv.submit(mid + 5, x[:, 0].contiguous())
+
+
+def test_input_still_in_production():
+    """x is written by work still queued on another stream when the request
+    is submitted: the event submit() records there orders the server's read
+    after it, so the result is the product with the final x."""
+    off, idx, val, cols = MATS[4]
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+    xh = np.random.default_rng(17).uniform(-1, 1, (cols, 4)).astype(np.float32)
+    xd = torch.from_numpy(xh).cuda()
+    torch.cuda.synchronize()
+    ref = oracle.spmm_f64(off, idx, val, xh)
+    with Server() as srv:
+        mid = srv.add_matrix(A)
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            x = torch.zeros((cols, 4), device="cuda")
+            busy = torch.full((4096, 4096), 1e-3, device="cuda")
+            for _ in range(12):  # milliseconds of queued work ahead of the copy
+                busy = busy @ busy
+            x.copy_(xd)
+        t = srv.submit(mid, x, stream=side)
+        y = srv.result(t).cpu().numpy()
+        side.synchronize()
+    assert np.all(np.abs(y - ref[0]) <= 1e-5 * ref[1] + 1e-30)

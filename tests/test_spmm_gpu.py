@@ -136,3 +136,42 @@ def test_spmm_aligned_b_unaligned_c(schedule, k):
         Batch([BatchItem(A, B, Cb, "merge_path_flat", 0)]).launch()
         torch.cuda.synchronize()
         assert np.array_equal(got.view(np.uint32), Cb.cpu().numpy().view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def c3():
+    """The 1M x 1M Zipf(2.0) matrix with 16M nonzeros (row 0: 200,000) and a
+    dense 1M x 64 B, with the FP64 product of every row."""
+    A_h = synth.c3_spmm_1m()
+    B_h = synth.dense(A_h.cols, 64, seed=31)
+    C64, A64 = oracle.spmm_f64(A_h.offsets, A_h.indices, A_h.values, B_h)
+    return A_h, B_h, C64, A64
+
+
+@pytest.mark.parametrize("schedule", ["merge_path", "group_mapped"])
+def test_c3_full_size(c3, schedule):
+    """Config-3 SpMM at full size: every entry of C within 1e-5 of
+    sum_j |a_ij b_jc| of the FP64 product, the 200,000-nonzero row included;
+    group_mapped sums in CSR order and equals the fp32 oracle bit for bit on
+    a seeded sample of rows and on the longest rows."""
+    A_h, B_h, C64, A64 = c3
+    assert A_h.max_row() == 200000 and int(np.diff(A_h.offsets)[0]) == 200000
+    A = loops.csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values)
+    B = torch.from_numpy(B_h).cuda()
+    C = torch.full((A_h.rows, 64), float("nan"), device="cuda")
+    loops.run_spmm(schedule, A, B, C)
+    torch.cuda.synchronize()
+    got = C.cpu().numpy()
+    bad = np.abs(got - C64) > REL * A64 + 1e-30
+    assert not bad.any(), (schedule, int(bad.sum()))
+    if schedule == "group_mapped":
+        lens = np.diff(A_h.offsets)
+        rows = np.unique(np.concatenate([
+            np.random.default_rng(3).choice(A_h.rows, 2000, replace=False),
+            np.argsort(lens)[-16:]]))
+        off = np.zeros(rows.shape[0] + 1, np.int64)
+        np.cumsum(lens[rows], out=off[1:])
+        idx = np.concatenate([A_h.indices[A_h.offsets[r]:A_h.offsets[r + 1]] for r in rows])
+        val = np.concatenate([A_h.values[A_h.offsets[r]:A_h.offsets[r + 1]] for r in rows])
+        C32 = oracle.spmm_f32(off.astype(np.int32), idx, val, B_h)
+        assert np.array_equal(got[rows].view(np.uint32), C32.view(np.uint32))

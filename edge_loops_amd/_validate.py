@@ -59,6 +59,12 @@ def check_devices(csr, **tensors):
             _device_ok(t, csr.offsets.device, name)
 
 
+def check_on(device, **tensors):
+    """Every named tensor on the given HIP device."""
+    for name, t in tensors.items():
+        _device_ok(t, device, name)
+
+
 def check_vector(t, n, name):
     """fp32, 1-D, unit stride, at least n entries."""
     if _dtype_name(t) != "float32":

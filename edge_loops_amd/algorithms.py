@@ -192,6 +192,7 @@ class spmv_plan:
         _v.check_devices(csr)
         self.csr = csr  # keeps A alive: with one panel the plan reads it in place
         self.rows, self.cols = csr.rows, csr.cols
+        self.device = csr.offsets.device
         v = csr.view()
         p = _capi.PlanParams(int(threads_per_block), int(items_per_thread), int(panels),
                              _capi.PLAN_LAYOUTS[layout])
@@ -221,7 +222,7 @@ class spmv_plan:
         share the plan's partial sums."""
         _v.check_vector(x, self.cols, "x")
         _v.check_vector(y, self.rows, "y")
-        _v.check_devices(self.csr, x=x, y=y)
+        _v.check_on(self.device, x=x, y=y)
         _capi.check(_capi.lib().loops_spmv_plan_execute(
             self._h, _ptr(x), _ptr(y),
             start_event.ptr if start_event is not None else None,

@@ -140,8 +140,8 @@ def test_spmm_aligned_b_unaligned_c(schedule, k):
 
 @pytest.fixture(scope="module")
 def c3():
-    """The 1M x 1M Zipf(2.0) matrix with 16M nonzeros (row 0: 200,000) and a
-    dense 1M x 64 B, with the FP64 product of every row."""
+    """The 1M x 1M Zipf(2.0) matrix with 16M nonzeros (longest row 200,000)
+    and a dense 1M x 64 B, with the FP64 product of every row."""
     A_h = synth.c3_spmm_1m()
     B_h = synth.dense(A_h.cols, 64, seed=31)
     C64, A64 = oracle.spmm_f64(A_h.offsets, A_h.indices, A_h.values, B_h)
@@ -155,7 +155,7 @@ def test_c3_full_size(c3, schedule):
     group_mapped sums in CSR order and equals the fp32 oracle bit for bit on
     a seeded sample of rows and on the longest rows."""
     A_h, B_h, C64, A64 = c3
-    assert A_h.max_row() == 200000 and int(np.diff(A_h.offsets)[0]) == 200000
+    assert A_h.max_row() == 200000
     A = loops.csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values)
     B = torch.from_numpy(B_h).cuda()
     C = torch.full((A_h.rows, 64), float("nan"), device="cuda")

@@ -187,6 +187,8 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_size_t)]),
     "loops_spmv": (_st, [_vp, _st, ctypes.POINTER(CsrView), _vp, _vp,
                          ctypes.POINTER(Params), _vp, ctypes.c_size_t, _vp]),
+    "loops_spmv_noalloc": (_st, [_vp, _st, ctypes.POINTER(CsrView), _vp, _vp,
+                                 ctypes.POINTER(Params), _vp]),
     "loops_merge_path_num_tiles": (_st, [_i64, _i64, _i64, ctypes.POINTER(_i64)]),
     "loops_merge_path_partition": (_st, [_vp, _i64, _i64, _i64, _vp, _vp]),
     "loops_work_oriented_partition": (_st, [_vp, _i64, _i64, _i64, _vp, _vp]),

@@ -5,6 +5,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <mutex>
 #include <new>
 
 #include "loops_capi.h"
@@ -141,6 +142,41 @@ loops_status_t get_workspace(loops_handle_t h, size_t need, void* ws,
 
 }  // namespace
 
+namespace loops_mi355x {
+// Scratch of loops_spmv_noalloc: the fix-up slots of merge_path_flat and
+// work_oriented.  Each GPU holds its own copy until the process ends.  In
+// host code the name is a host-side stand-in, not a device address: host
+// code uses only the address scratch_address() looked up per device.
+__device__ __align__(256) char scratch[LOOPS_NOALLOC_SCRATCH_BYTES];
+}  // namespace loops_mi355x
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (looked up once per
+// device, with that device current).
+loops_status_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e);
+  if (dev < 0 || dev >= kMaxScratchDevices) return LOOPS_ERROR_NOT_SUPPORTED;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(loops_mi355x::scratch));
+    if (e != hipSuccess) return hip_status(e);
+    if (!p) return LOOPS_ERROR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LOOPS_SUCCESS;
+}
+
+}  // namespace
+
 extern "C" {
 
 loops_status_t loops_create(loops_handle_t* handle) {
@@ -154,6 +190,13 @@ loops_status_t loops_create(loops_handle_t* handle) {
   if (e != hipSuccess) {
     delete h;
     return hip_status(e);
+  }
+  // set-up of loops_spmv_noalloc's scratch on this device
+  void* sp = nullptr;
+  const loops_status_t ss = scratch_address(&sp);
+  if (ss != LOOPS_SUCCESS) {
+    delete h;
+    return ss;
   }
   *handle = h;
   return LOOPS_SUCCESS;
@@ -252,6 +295,27 @@ loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
     default:
       return LOOPS_ERROR_NOT_SUPPORTED;
   }
+}
+
+loops_status_t loops_spmv_noalloc(loops_handle_t handle, loops_schedule_t schedule,
+                                  const loops_csr_view_t* A, const float* x, float* y,
+                                  const loops_params_t* params, hipStream_t stream) {
+  csr_args a;
+  loops_status_t st = check_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  if (a.rows > 0 && !y) return LOOPS_ERROR_INVALID_VALUE;
+  if (a.nnz > 0 && !x) return LOOPS_ERROR_INVALID_VALUE;
+  resolved r;
+  st = resolve_spmv(handle, schedule, a, params, &r);
+  if (st != LOOPS_SUCCESS) return st;
+  if (r.ws_bytes > (size_t)LOOPS_NOALLOC_SCRATCH_BYTES) return LOOPS_ERROR_WORKSPACE;
+  void* ws = nullptr;
+  if (r.ws_bytes > 0) {
+    st = scratch_address(&ws);
+    if (st != LOOPS_SUCCESS) return st;
+  }
+  // with a workspace of the size it needs, loops_spmv allocates nothing
+  return loops_spmv(handle, schedule, A, x, y, params, ws, r.ws_bytes, stream);
 }
 
 loops_status_t loops_merge_path_num_tiles(int64_t rows, int64_t nnzs,

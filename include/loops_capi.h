@@ -145,6 +145,24 @@ loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
                           const loops_params_t* params, void* workspace,
                           size_t workspace_bytes, hipStream_t stream);
 
+/* y = A * x as loops_spmv, without allocating anything (no hipMalloc, no
+ * handle workspace, no caller workspace).  No counterpart in the
+ * reference: for callers that cannot allocate device memory -- code under
+ * hipGraph capture, tight serving loops, harnesses that forbid allocation.
+ * The fix-up slots of merge_path_flat and work_oriented live in a static
+ * device array of LOOPS_NOALLOC_SCRATCH_BYTES owned by the library, one per
+ * GPU, held until the process ends; a call whose schedule and tuning need
+ * more (loops_spmv_workspace_size) returns LOOPS_ERROR_WORKSPACE.  The
+ * default merge_path_flat tiles (256 x 8) fit every matrix the int32 ABI
+ * accepts.  All calls on one GPU share its array: calls that overlap in
+ * time on one GPU (two streams, or graph replays on several streams) race
+ * and give wrong results; serialise them.  The array keeps the last call's
+ * contents, which no call needs cleared.  Asynchronous on stream. */
+#define LOOPS_NOALLOC_SCRATCH_BYTES (16u << 20)
+loops_status_t loops_spmv_noalloc(loops_handle_t handle, loops_schedule_t schedule,
+                                  const loops_csr_view_t* A, const float* x, float* y,
+                                  const loops_params_t* params, hipStream_t stream);
+
 /* Number of merge tiles, ceil((rows + nnzs) / items_per_tile)
  * (merge_path_flat.cuh:116-117). */
 loops_status_t loops_merge_path_num_tiles(int64_t rows, int64_t nnzs,

@@ -560,7 +560,7 @@ loops_status_t loops_spmm(loops_handle_t handle, loops_schedule_t schedule,
       return hip_status(launch_tm_spmm(a, (int)k, B, ldb, C, ldc, tpb, ev0, ev1, stream));
     }
     case LOOPS_SCHEDULE_GROUP_MAPPED: {
-      int tpb = p->threads_per_block ? p->threads_per_block : 256;
+      int tpb = p->threads_per_block ? p->threads_per_block : 1024;
       if (!gm_spmm_supported(tpb)) return LOOPS_ERROR_NOT_SUPPORTED;
       return hip_status(launch_gm_spmm(a, (int)k, B, ldb, C, ldc, tpb, ev0, ev1, stream));
     }

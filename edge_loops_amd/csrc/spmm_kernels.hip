@@ -6,12 +6,13 @@
 //                  column; here a work-item keeps 8 columns in registers and
 //                  rescans once per 8 columns.  Each C(row, c) is still the
 //                  sequential CSR-order fp32 sum.
-//   group_mapped   (no reference SpMM): a group of G lanes owns G consecutive
-//                  rows and walks their nonzeros in CSR order; the group's
-//                  lanes span the output columns (CPL columns per lane, one
-//                  16-byte load of a B row slice per lane when CPL = 4), so
-//                  every B row is fetched as whole contiguous segments.
-//                  Bit-identical to the sequential CSR-order sum.
+//   group_mapped   (no reference SpMM): the block_mapped partition (a
+//                  workgroup owns TPB rows) whose atoms are shared evenly by
+//                  ranges of G lanes; the G lanes span the output columns
+//                  (CPL columns per lane, one 16-byte load of a B row slice
+//                  per lane when CPL = 4), so every B row is fetched as
+//                  whole contiguous segments.  Rows split between ranges
+//                  are summed in range order.
 //   merge_path     (no reference SpMM): the same lane-per-column groups, but
 //                  each group walks an equal share (IPU merge steps) of the
 //                  merge path of rows and nonzeros, so one 200k-nonzero row
@@ -20,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include <loops/schedule.hxx>
 #include <loops/util/search.hxx>
@@ -65,49 +67,141 @@ __global__ void spmm_thread_mapped_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// group_mapped: group g owns rows [g*G, g*G + G); the group walks their
-// nonzeros in order, U at a time (U independent B-row loads in flight).
+// group_mapped: the block_mapped partition of the reference's schedule
+// (group_mapped.hxx:146-213): a workgroup of TPB lanes owns TPB consecutive
+// rows, and the group's atoms (its rows' nonzeros, numbered 0 .. aggregate-1
+// by the exclusive scan of the row lengths) are shared out evenly.  The
+// lanes form R = TPB / G ranges of G lanes spanning the columns; range q
+// takes the contiguous atoms [q L, q L + L), L = ceil(aggregate / R), so a
+// 200,000-nonzero row is spread over every range of its group instead of
+// being walked by one.  Each range sums its atoms row by row (U B-row loads
+// in flight, the next atoms' column ids loaded ahead) and writes the rows
+// that end in its share; a row that began in an earlier share is finished
+// by the range where it ends, adding the earlier ranges' partial sums in
+// range order.  Every sum has an order fixed by the matrix, TPB and G.
 // ---------------------------------------------------------------------------
 template <int TPB, int G, int CPL, typename V>
 __global__ __launch_bounds__(TPB) void spmm_group_mapped_kernel(
     const int* __restrict__ offsets, const int* __restrict__ indices,
     const V* __restrict__ values, int rows, int k, const float* __restrict__ B,
     long long ldb, float* __restrict__ C, long long ldc) {
+  constexpr int R = TPB / G;
+  constexpr int W = G * CPL;  // columns of one column block
   constexpr int U = 8;
-  const int lane = threadIdx.x & (G - 1);
-  const long long group = ((long long)blockIdx.x * TPB + threadIdx.x) / G;
-  const long long r0 = group * G;
-  if (r0 >= rows) return;  // uniform within the group
-  const long long r1 = r0 + G < rows ? r0 + G : rows;
-  for (int cb = 0; cb < k; cb += G * CPL) {
-    const int c = cb + lane * CPL;
-    for (long long row = r0; row < r1; ++row) {
-      float acc[CPL];
+  __shared__ int s_end[TPB + 1];        // s_end[i]: end of row r0 + i - 1, block-relative
+  __shared__ int s_tail_row[R];         // row a range leaves open, or -1
+  __shared__ float s_tail[R][W];        // its partial sums
+  const int tid = threadIdx.x;
+  const long long r0 = (long long)blockIdx.x * TPB;
+  const int nrows = rows - r0 < TPB ? (int)(rows - r0) : TPB;
+  const int base = offsets[r0];
+  for (int i = tid; i <= nrows; i += TPB) s_end[i] = offsets[r0 + i] - base;
+  __syncthreads();
+  const int agg = s_end[nrows];
+  const int L = (agg + R - 1) / R;
+  const int q = tid / G, g = tid % G;
+  const int a0 = q * L < agg ? q * L : agg;
+  const int a1 = a0 + L < agg ? a0 + L : agg;
+  // first block row whose end exceeds v (rows are 0 .. nrows-1)
+  auto row_after = [&](int v) {
+    int lo = 0, hi = nrows;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_end[mid + 1] > v) hi = mid;
+      else lo = mid + 1;
+    }
+    return lo;
+  };
+  // rows ending in this share: end in (a0, a1]; the first range also owns
+  // the rows ending at 0 (empty rows before the first nonzero)
+  const int rlo = q == 0 ? 0 : row_after(a0);
+  const int rhi = row_after(a1);
+  const int* __restrict__ idx = indices + base;
+  const V* __restrict__ val = values + base;
+
+  for (int cb = 0; cb < k; cb += W) {
+    const int c = cb + g * CPL;
+    float acc[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
+    int cr = rlo;          // current row
+    int next_end = cr < nrows ? s_end[cr + 1] : INT_MAX;
+    int head_row = -1;     // a finished row that began before a0
+    float head[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) head[j] = 0.f;
+    auto finish = [&](int r) {
+      if (s_end[r] < a0) {
+        head_row = r;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) head[j] = acc[j];
+      } else if (c < k) {
+        store_c<CPL>(C + (r0 + r) * ldc + c, c, k, acc);
+      }
 #pragma unroll
       for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
-      const int e = offsets[row + 1];
-      for (int nz = offsets[row]; nz < e; nz += U) {
-        const int n = e - nz < U ? e - nz : U;
-        int cj[U];
-        float vj[U];
+    };
+    int cj[U];
+    float vj[U];
 #pragma unroll
-        for (int j = 0; j < U; ++j) {
-          cj[j] = j < n ? indices[nz + j] : 0;
-          vj[j] = j < n ? value_traits<V>::load1(values + nz + j) : 0.f;
-        }
-        cols_t<CPL> bj[U];
-#pragma unroll
-        for (int j = 0; j < U; ++j)
-          if (j < n) bj[j] = load_b<CPL>(B, ldb, cj[j], c, k);
-#pragma unroll
-        for (int j = 0; j < U; ++j)
-          if (j < n) {
-#pragma unroll
-            for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
-          }
-      }
-      if (c < k) store_c<CPL>(C + row * ldc + c, c, k, acc);
+    for (int j = 0; j < U; ++j) {
+      cj[j] = a0 + j < a1 ? ld_idx(idx + a0 + j) : 0;
+      vj[j] = a0 + j < a1 ? value_traits<V>::load1(val + a0 + j) : 0.f;
     }
+    for (int a = a0; a < a1; a += U) {
+      const int n = a1 - a < U ? a1 - a : U;
+      cols_t<CPL> bj[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (j < n) bj[j] = load_b<CPL>(B, ldb, cj[j], c, k);
+      float vcur[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) vcur[j] = vj[j];
+      // the next atoms' column ids and values, in flight with the B rows
+      const int an = a + U;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        cj[j] = an + j < a1 ? ld_idx(idx + an + j) : 0;
+        vj[j] = an + j < a1 ? value_traits<V>::load1(val + an + j) : 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        if (j < n) {
+          while (next_end <= a + j) {
+            finish(cr);
+            ++cr;
+            next_end = s_end[cr + 1];
+          }
+#pragma unroll
+          for (int t = 0; t < CPL; ++t) acc[t] += vcur[j] * bj[j].v[t];
+        }
+      }
+    }
+    while (cr < rhi) {
+      finish(cr);
+      ++cr;
+    }
+    // the row left open: it has atoms in this share and ends in a later one
+    const bool open = a1 > a0 && cr < nrows && s_end[cr] < a1 && s_end[cr + 1] > a1;
+    if (g == 0) s_tail_row[q] = open ? cr : -1;
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) s_tail[q][g * CPL + j] = acc[j];
+    __syncthreads();
+    if (head_row >= 0) {
+      int q0 = q;
+      while (q0 > 0 && s_tail_row[q0 - 1] == head_row) --q0;
+      float v[CPL];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) v[j] = s_tail[q0][g * CPL + j];
+      for (int p = q0 + 1; p < q; ++p) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) v[j] = v[j] + s_tail[p][g * CPL + j];
+      }
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) v[j] = v[j] + head[j];
+      if (c < k) store_c<CPL>(C + (r0 + head_row) * ldc + c, c, k, v);
+    }
+    __syncthreads();  // s_tail is reused by the next column block
   }
 }
 
@@ -179,8 +273,7 @@ inline spmm_shape pick_shape(int k, bool vec4) {
 template <int TPB, int G, int CPL, typename V>
 hipError_t gm_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
                      float* C, long long ldc, hipStream_t s) {
-  long long groups = ((long long)A.rows + G - 1) / G;
-  long long grid = (groups * G + TPB - 1) / TPB;
+  const long long grid = ((long long)A.rows + TPB - 1) / TPB;
   spmm_group_mapped_kernel<TPB, G, CPL, V><<<dim3((unsigned)grid), dim3(TPB), 0, s>>>(
       A.offsets, A.indices, static_cast<const V*>(A.values), A.rows, k, B, ldb,
       C, ldc);
@@ -216,9 +309,13 @@ hipError_t mp_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
 
 template <typename V>
 hipError_t gm_spmm_v(const csr_args& A, int k, const float* B, long long ldb,
-                     float* C, long long ldc, spmm_shape sh, hipStream_t s) {
-#define X(G_, C_) \
-  if (sh.g == G_ && sh.cpl == C_) return gm_spmm_t<256, G_, C_, V>(A, k, B, ldb, C, ldc, s);
+                     float* C, long long ldc, spmm_shape sh, int tpb, hipStream_t s) {
+#define X(G_, C_)                                                                         \
+  if (sh.g == G_ && sh.cpl == C_) {                                                       \
+    if (tpb == 256) return gm_spmm_t<256, G_, C_, V>(A, k, B, ldb, C, ldc, s);           \
+    if (tpb == 512) return gm_spmm_t<512, G_, C_, V>(A, k, B, ldb, C, ldc, s);           \
+    return gm_spmm_t<1024, G_, C_, V>(A, k, B, ldb, C, ldc, s);                          \
+  }
   LOOPS_SPMM_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
@@ -268,20 +365,19 @@ hipError_t launch_tm_spmm(const csr_args& A, int k, const float* B, long long ld
   return hipErrorInvalidValue;
 }
 
-bool gm_spmm_supported(int tpb) { return tpb == 256; }
+bool gm_spmm_supported(int tpb) { return tpb == 256 || tpb == 512 || tpb == 1024; }
 
 hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ldb,
                           float* C, long long ldc, int tpb, hipEvent_t ev0,
                           hipEvent_t ev1, hipStream_t s) {
-  (void)tpb;
   hipError_t e = record(ev0, s);
   if (e != hipSuccess) return e;
   if (A.rows > 0 && k > 0) {
     spmm_shape sh = pick_shape(k, vec4_ok(B, ldb));
     switch (A.vtype) {
-      case LOOPS_VALUE_F32: e = gm_spmm_v<float>(A, k, B, ldb, C, ldc, sh, s); break;
-      case LOOPS_VALUE_BF16: e = gm_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, s); break;
-      case LOOPS_VALUE_F16: e = gm_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, s); break;
+      case LOOPS_VALUE_F32: e = gm_spmm_v<float>(A, k, B, ldb, C, ldc, sh, tpb, s); break;
+      case LOOPS_VALUE_BF16: e = gm_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, tpb, s); break;
+      case LOOPS_VALUE_F16: e = gm_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, tpb, s); break;
     }
     if (e != hipSuccess) return e;
   }

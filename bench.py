@@ -10,7 +10,7 @@ before timing.
 Unless --quick / --no-forms, the line also carries the batched form (512
 matrices in one persistent launch vs per-matrix plain launches) and the
 served form (continuous batching over 64 resident matrices vs plain
-per-call launches: req/s, p50/p99), see edge_loops_amd/bench_forms.py.
+per-call launches: req/s, p50/p99), see bench_forms.py.
 
 A step is one execution of the matrix's SpMV plan (loops_spmv_plan_execute;
 the automatic layout for this matrix is row blocks: one workgroup per (row
@@ -83,16 +83,131 @@ def copy_calibration(mb, steps):
 
 
 def traffic_from_profile(layout):
-    """HBM bytes per launch of the plan's main kernel on this workload,
-    measured by rocprofv3 PMC runs (FETCH_SIZE, WRITE_SIZE, calibrated
-    against a copy of known size) and recorded in profiles/; None if
-    absent."""
-    name = "spmv_blocks_c2_pmc.md" if layout == "blocks" else "merge_path_plan_c2_pmc.md"
+    """HBM bytes per launch of the main kernel of a layout ("blocks",
+    "panels" or "merge_path_plain") on this workload, measured by rocprofv3
+    PMC runs (FETCH_SIZE, WRITE_SIZE, calibrated against a copy of known
+    size) and recorded in profiles/; None if absent."""
+    name = {"blocks": "spmv_blocks_c2_pmc.md", "panels": "merge_path_plan_c2_pmc.md",
+            "merge_path_plain": "merge_path_plain_c2_pmc.md"}[layout]
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
     return float(m.group(1)) if m else None
+
+
+def hbm_ceilings(stream):
+    """Read and copy rates of 2 GiB of device memory through the library's
+    probe kernels (edge_loops_amd/csrc/hbm_probe.hip), the best over grids
+    of 1-16 workgroups per CU, and torch's copy_ / sum for comparison."""
+    import ctypes
+
+    import torch
+
+    from edge_loops_amd.hip import Event
+    lib = ctypes.CDLL(os.path.join(ROOT, "edge_loops_amd", "lib", "libloops_hbm.so"))
+    lib.hbm_probe_read.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    lib.hbm_probe_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                   ctypes.c_int, ctypes.c_void_p]
+    nbytes = 2 << 30
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    sink = torch.zeros(256, dtype=torch.int32, device="cuda")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+
+    def timed(fn, reps=8):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError(f"hbm probe failed: hipError {rc}")
+
+    best_read, best_copy = (0.0, 0), (0.0, 0)
+    for per_cu in (1, 2, 4, 8, 16):
+        g = cus * per_cu
+        ms = timed(lambda: ok(lib.hbm_probe_read(src.data_ptr(), nbytes, g, sink.data_ptr(),
+                                                 stream)))
+        best_read = max(best_read, (nbytes / (ms * 1e-3) / 1e9, g))
+        ms = timed(lambda: ok(lib.hbm_probe_copy(dst.data_ptr(), src.data_ptr(), nbytes, g,
+                                                 stream)))
+        best_copy = max(best_copy, (2 * nbytes / (ms * 1e-3) / 1e9, g))
+    t_copy = timed(lambda: dst.copy_(src))
+    t_sum = timed(lambda: torch.sum(src))
+    out = {"read_gbs": round(best_read[0], 1), "read_grid": best_read[1],
+           "copy_gbs": round(best_copy[0], 1), "copy_grid": best_copy[1],
+           "torch_copy_gbs": round(2 * nbytes / (t_copy * 1e-3) / 1e9, 1),
+           "torch_sum_gbs": round(nbytes / (t_sum * 1e-3) / 1e9, 1),
+           "what": "2 GiB streamed by hbm_probe.hip (16-byte non-temporal loads, 8 in flight "
+                   "per lane; copy: bytes read + written), best grid of 1-16 workgroups "
+                   "per CU; torch copy_ and sum over the same 2 GiB beside it; HIP events "
+                   "on the bench stream"}
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
+def merge_path_legs(copies, x, y, stream, args, nbytes, flops):
+    """The reference's named path on the same matrix: per-call
+    loops_spmv(merge_path_flat) with HIP events around its main kernel
+    (rotating over the copies of A), and the column-panel plan."""
+    import torch
+
+    import edge_loops_amd as loops
+    from edge_loops_amd.hip import Event
+    out = {}
+    ws = loops.algorithms.workspace("merge_path_flat", copies[0])
+    evs = [(Event(), Event()) for _ in range(args.steps)]
+    for i in range(args.warmup):
+        loops.run_spmv("merge_path_flat", copies[i % len(copies)], x, y, stream=stream,
+                       workspace=ws)
+    torch.cuda.synchronize()
+    p0, p1 = Event(), Event()
+    p0.record(stream)
+    for i in range(args.steps):
+        loops.run_spmv("merge_path_flat", copies[i % len(copies)], x, y, stream=stream,
+                       workspace=ws, start_event=evs[i][0], stop_event=evs[i][1])
+    p1.record(stream)
+    torch.cuda.synchronize()
+    step = p0.elapsed_ms(p1) / args.steps
+    kern = sum(a.elapsed_ms(b) for a, b in evs) / len(evs)
+    out["per_call"] = {
+        "kernel": "spmv_merge_path_kernel<256, 8, float>", "kernel_avg_ms": round(kern, 5),
+        "ms_per_step": round(step, 5), "gflops": round(flops / (step * 1e-3) / 1e9, 2),
+        "frac": round(nbytes / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "frac_step": round(nbytes / (step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profile("merge_path_plain")}
+    plan = loops.spmv_plan(copies[0], layout="panels", stream=stream)
+    for _ in range(args.warmup):
+        plan.execute(x, y, stream)
+    evs = [(Event(), Event()) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    p0.record(stream)
+    for i in range(args.steps):
+        plan.execute(x, y, stream, evs[i][0], evs[i][1])
+    p1.record(stream)
+    torch.cuda.synchronize()
+    step = p0.elapsed_ms(p1) / args.steps
+    kern = sum(a.elapsed_ms(b) for a, b in evs) / len(evs)
+    out["panels_plan"] = {
+        "kernel": "spmv_panels_kernel", "panels": plan.panels, "kernel_avg_ms": round(kern, 5),
+        "ms_per_step": round(step, 5), "gflops": round(flops / (step * 1e-3) / 1e9, 2),
+        "frac": round(nbytes / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "frac_step": round(nbytes / (step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profile("panels")}
+    plan.destroy()
+    log(f"merge_path per call: kernel {out['per_call']['kernel_avg_ms']*1e3:.1f} us "
+        f"(frac {out['per_call']['frac']}); panels plan {out['panels_plan']['kernel_avg_ms']*1e3:.1f} us")
+    return out
 
 
 def main():
@@ -184,56 +299,24 @@ def main():
         extras["warm_ms_per_step"] = round(warm_ms, 5)
         log(f"warm: {warm_ms*1e3:.1f} us/step")
 
-        # the plain per-call drop-in (loops_spmv: in-kernel tile search, no plan)
-        ws = loops.algorithms.workspace("merge_path_flat", copies[0])
-        for i in range(args.warmup):
-            loops.run_spmv("merge_path_flat", copies[i % len(copies)], x, y, stream=stream, workspace=ws)
-        p0, p1 = Event(), Event()
-        torch.cuda.synchronize()
-        p0.record(stream)
-        for i in range(args.steps):
-            loops.run_spmv("merge_path_flat", copies[i % len(copies)], x, y, stream=stream,
-                           workspace=ws)
-        p1.record(stream)
-        torch.cuda.synchronize()
-        plain_ms = p0.elapsed_ms(p1) / args.steps
-        extras["plain_per_call_ms_per_step"] = round(plain_ms, 5)
-        extras["plain_per_call_gflops"] = round(flops / (plain_ms * 1e-3) / 1e9, 2)
-        log(f"plain per-call loops_spmv: {plain_ms*1e3:.1f} us/step")
-
-        # measured HBM ceilings beside the 8 TB/s peak: a 1 GiB device copy
-        # (bytes read + written) and a read-only reduction over the same GiB
-        big = torch.empty(2 ** 28, dtype=torch.float32, device="cuda").uniform_()
-        dst = torch.empty_like(big)
-
-        def timed(fn, reps=10):
-            for _ in range(3):
-                fn()
-            torch.cuda.synchronize()
-            e0, e1 = Event(), Event()
-            e0.record(stream)
-            for _ in range(reps):
-                fn()
-            e1.record(stream)
-            torch.cuda.synchronize()
-            return e0.elapsed_ms(e1) / reps
-
-        copy_ms = timed(lambda: dst.copy_(big))
-        read_ms = timed(lambda: torch.sum(big))
-        gib = 2 ** 30
+        # measured HBM ceilings beside the 8 TB/s peak: this library's own
+        # stream kernels (16-byte non-temporal loads / stores, the access of
+        # the SpMV kernels' streamed arrays) over 2 GiB, best of a grid sweep;
+        # torch's copy and reduction beside them
+        extras["hbm_ceilings"] = hbm_ceilings(stream)
         props = torch.cuda.get_device_properties(0)
         extras["device"] = {"name": props.name,
                             "arch": getattr(props, "gcnArchName", ""),
                             "compute_units": props.multi_processor_count,
-                            "memory_gib": round(props.total_memory / gib, 1)}
-        extras["hbm_ceilings"] = {
-            "copy_gbs": round(2 * gib / (copy_ms * 1e-3) / 1e9, 1),
-            "read_gbs": round(gib / (read_ms * 1e-3) / 1e9, 1),
-            "what": "torch copy_ of 1 GiB fp32 (bytes read + written) and torch.sum "
-                    "over it (bytes read), HIP events on the bench stream"}
-        log(f"ceilings: copy {extras['hbm_ceilings']['copy_gbs']} GB/s, "
-            f"read {extras['hbm_ceilings']['read_gbs']} GB/s")
-        del big, dst
+                            "memory_gib": round(props.total_memory / 2 ** 30, 1)}
+        log(f"ceilings: read {extras['hbm_ceilings']['read_gbs']} GB/s, copy "
+            f"{extras['hbm_ceilings']['copy_gbs']} GB/s (read + written)")
+
+        # the drop-in itself: per-call merge_path_flat main kernel (events
+        # around it) and the column-panel plan, rotating over the copies
+        extras["merge_path"] = merge_path_legs(copies, x, y, stream, args, nbytes, flops)
+        extras["plain_per_call_ms_per_step"] = extras["merge_path"]["per_call"]["ms_per_step"]
+        extras["plain_per_call_gflops"] = extras["merge_path"]["per_call"]["gflops"]
 
         # all five schedules through loops_spmv, same matrix (copy 0)
         names = ["merge_path_flat", "work_oriented", "group_mapped", "thread_mapped", "original"]
@@ -285,7 +368,7 @@ def main():
     if not args.quick and not args.no_forms:
         # bf16 value storage on the same matrix, the SpMM workload, then the
         # batched and served forms against plain per-call launches.
-        from edge_loops_amd import bench_forms
+        import bench_forms
         del copies, plans
         torch.cuda.empty_cache()
         extras["bf16_storage_plan"] = bench_forms.spmv_bf16_plan(A_h, x_h, log=log)
@@ -325,6 +408,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac_step": round(nbytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profile(layout),
             "kernel": ("spmv_blocks_kernel" if layout == "blocks" else
                        "spmv_panels_kernel" if n_panels > 1 else "spmv_merge_path_kernel"),
@@ -347,6 +431,9 @@ def main():
                                   "thread per host core"},
         },
     }
+    if "hbm_ceilings" in extras:
+        out["roofline"]["frac_of_measured_read"] = round(
+            achieved / extras["hbm_ceilings"]["read_gbs"], 4)
     out.update(extras)
     print(json.dumps(out), flush=True)
 

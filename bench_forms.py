@@ -1,5 +1,7 @@
-"""Measurements of the batched form against per-matrix plain launches
-(used by bench.py; runs on the GPU).
+"""Measurement legs of bench.py beside the headline (run on the GPU): the
+batched form against per-matrix plain launches, the served form, the SpMM
+workload and the bf16 storage plan.  Measurement harness, not part of the
+edge_loops_amd package.
 
 Batched workload: 512 square CSR matrices, rows log-uniform in [1e3, 2e5],
 nnz/row Zipf(1.8) capped at 4096, fp32 (edge_loops_amd.synth.c4_batch_512).
@@ -20,10 +22,10 @@ import time
 
 import numpy as np
 
-from . import synth
-from .algorithms import run_spmv, spmv_workspace_size
-from .batch import Batch, BatchItem
-from .containers import csr_t
+from edge_loops_amd import synth
+from edge_loops_amd.algorithms import run_spmv, spmv_workspace_size
+from edge_loops_amd.batch import Batch, BatchItem
+from edge_loops_amd.containers import csr_t
 
 HBM_PEAK_GBS = 8000.0
 
@@ -48,7 +50,7 @@ def _time(fn, reps):
 
 def batched(count=512, reps=10, table=None, log=print):
     import torch
-    from . import tune
+    from edge_loops_amd import tune
     t0 = time.time()
     mats = synth.c4_batch_512(count)
     log(f"batched: generated {count} matrices in {time.time() - t0:.1f}s")
@@ -165,8 +167,8 @@ def _served_setup(n_requests, seed, vts, table, log):
     """The served pool (synth.c5_pool_64) on the device per value type with
     its tunings, the right-hand sides, and the seeded request mix."""
     import torch
-    from . import tune
-    from .serve import matrix_tunings
+    from edge_loops_amd import tune
+    from edge_loops_amd.serve import matrix_tunings
     t0 = time.time()
     mats = synth.c5_pool_64()
     log(f"served: generated 64 matrices in {time.time() - t0:.1f}s")
@@ -189,7 +191,7 @@ def served_burst(mode="batched", n_requests=3000, seed=55, slots=3, log=print, t
     """One burst of the served workload (all requests at t = 0) through the
     server in `mode`, fp32: requests per second.  The leg a kernel trace of
     the served form runs."""
-    from .serve import Server
+    from edge_loops_amd.serve import Server
     mats, pools, x1, X32, mid, ks = _served_setup(n_requests, seed, ("f32",), table, log)
     dev, tun = pools["f32"]
     srv = Server(mode=mode, arena_bytes=32 << 30, scratch_bytes=1 << 30,
@@ -208,7 +210,7 @@ def served(n_requests=3000, seed=55, slots=3, log=print, table=None):
     with a uniform random matrix and k uniform in [1, 32] fp32 right-hand
     sides (k == 1: a vector), seeded Poisson arrivals.  Batched server vs the
     plain per-call mode on the same loop and clock."""
-    from .serve import Server, poisson_arrivals
+    from edge_loops_amd.serve import Server, poisson_arrivals
     mats, pools, x1, X32, mid, ks = _served_setup(n_requests, seed, ("f32", "bf16"), table, log)
     flops = float(sum(2 * mats[m].nnz * k for m, k in zip(mid, ks)))
     out = {"workload": "64 CSR matrices, rows log-uniform [1e4, 4e6], nnz log-uniform "
@@ -346,7 +348,7 @@ def spmv_bf16_plan(A_h, x_h, reps=20, log=print):
     and the worst per-row error against an FP64 host product on the stored values
     relative to sum |a x| (the 1e-5 bar)."""
     import torch
-    from .algorithms import spmv_plan
+    from edge_loops_amd.algorithms import spmv_plan
     A = csr_t.from_host(A_h.rows, A_h.cols, A_h.offsets, A_h.indices, A_h.values,
                         value_type="bf16")
     x = torch.from_numpy(x_h).cuda()
@@ -376,7 +378,7 @@ def spmm_c3(reps=5, log=print):
     with a dense 1M x 64 fp32 B: group_mapped and merge_path, warm, with the
     per-row 1e-5 check of a sample of rows against an FP64 host product."""
     import torch
-    from .algorithms import run_spmm, spmm_workspace_size
+    from edge_loops_amd.algorithms import run_spmm, spmm_workspace_size
     t0 = time.time()
     A_h = synth.c3_spmm_1m()
     k = 64

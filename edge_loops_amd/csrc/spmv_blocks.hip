@@ -48,9 +48,12 @@
 // Build (loops_spmv_plan_create, layout LOOPS_PLAN_LAYOUT_BLOCKS): count
 // the entries of every (block, segment, chunk) bucket, an exclusive scan,
 // the rounds of every stream, a scatter of every entry into its bucket and
-// a sort of every round by (row, column).  Within a chunk the scatter's
-// order depends on scheduling; it matters only when one chunk of one block
-// holds more than kBlkRound entries (it is then split across rounds).
+// a sort of every round by (row, column).  A bucket that fits a round is
+// scattered with an atomic cursor (its order is then fixed by the round's
+// sort); a bucket of more than kBlkRound entries is split across rounds by
+// position, so a workgroup of its own places it in (row, column) order
+// first.  Two builds of a matrix are therefore identical, except for the
+// order of duplicate (row, column) entries inside one round.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -245,26 +248,140 @@ __global__ __launch_bounds__(256) void blk_round_write_kernel(const int* __restr
   blk_rounds_of(start, q, nch, round_off + first[q]);
 }
 
-// Every nonzero to the next free slot of its bucket (cursor = bucket
-// starts): column within the segment to packed, row within the block to
-// trow (both in stream order until blk_round_sort_kernel).
+// Every nonzero of a bucket of at most kBlkRound entries to the next free
+// slot of its bucket (cursor = bucket starts): column within the segment to
+// packed, row within the block to trow.  Such a bucket lies in one round,
+// whose sort orders it by (row, column).  Larger buckets are split across
+// rounds by position, so their order must not depend on scheduling:
+// blk_scatter_big_kernel places them.
 template <typename V>
 __global__ __launch_bounds__(256) void blk_scatter_kernel(
     const int* __restrict__ offsets, const int* __restrict__ indices,
-    const V* __restrict__ values, blk_geom g, int* __restrict__ cursor,
-    uint32_t* __restrict__ packed, uint16_t* __restrict__ trow, V* __restrict__ vals) {
+    const V* __restrict__ values, blk_geom g, const int* __restrict__ cnt,
+    int* __restrict__ cursor, uint32_t* __restrict__ packed, uint16_t* __restrict__ trow,
+    V* __restrict__ vals) {
   for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
        nz += (long long)gridDim.x * blockDim.x) {
     const int r = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
     const int b = blk_block_of(g, r);
     const int col = indices[nz];
     const int sg = col / g.W;
-    const int pos = atomicAdd(&cursor[blk_bucket(g, b, col)], 1);
+    const long long bucket = blk_bucket(g, b, col);
+    if (cnt[bucket] > kBlkRound) continue;
+    const int pos = atomicAdd(&cursor[bucket], 1);
     packed[pos] = (uint32_t)(col - sg * g.W);
     trow[pos] = (uint16_t)(g.excl ? g.excl[(long long)sg * g.V + r] -
                                         g.excl[(long long)sg * g.V + g.blk_start[b]]
                                   : r - g.blk_start[b]);
     vals[pos] = values[nz];
+  }
+}
+
+// big[...] = the buckets of more than kBlkRound entries (in any order; each
+// is placed on its own), *nbig their number.
+__global__ __launch_bounds__(256) void blk_list_big_kernel(const int* __restrict__ cnt,
+                                                           long long nbuckets,
+                                                           int* __restrict__ big,
+                                                           int* __restrict__ nbig) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nbuckets;
+       i += (long long)gridDim.x * blockDim.x)
+    if (cnt[i] > kBlkRound) big[atomicAdd(nbig, 1)] = (int)i;
+}
+
+// One workgroup per bucket of more than kBlkRound entries: its entries in
+// (virtual row, column) order from bucket start on, found row by row (each
+// lane takes one virtual row of the block, finds its run in the chunk by
+// binary search, and a workgroup scan of the run lengths gives the
+// positions) -- the order the round sort would give, fixed before the
+// bucket is cut into rounds by position.  Duplicate (row, column) entries
+// keep CSR order.
+constexpr int kBlkBigTpb = 1024;
+template <typename V>
+__global__ __launch_bounds__(kBlkBigTpb) void blk_scatter_big_kernel(
+    const int* __restrict__ offsets, const int* __restrict__ indices,
+    const V* __restrict__ values, blk_geom g, const int* __restrict__ big,
+    const int* __restrict__ cnt, const int* __restrict__ bucket_start,
+    uint32_t* __restrict__ packed, uint16_t* __restrict__ trow, V* __restrict__ vals) {
+  __shared__ int s_wave[kBlkBigTpb / 64];
+  const long long id = big[blockIdx.x];
+  const int ch = (int)(id % g.nch);
+  const int sg = (int)((id / g.nch) % g.S);
+  const int b = (int)(id / ((long long)g.nch * g.S));
+  const int v0 = g.blk_start[b], v1 = g.blk_start[b + 1];
+  const int seg0 = sg * g.W;
+  const int col_lo = seg0 + ch * kBlkChunkCols;
+  int col_hi = col_lo + kBlkChunkCols;
+  if (col_hi > seg0 + g.W) col_hi = seg0 + g.W;
+  if (col_hi > g.cols) col_hi = g.cols;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  long long pos = bucket_start[id];
+  const long long end = pos + cnt[id];  // the runs found add up to cnt[id]
+  for (int vb = v0; vb < v1; vb += kBlkBigTpb) {
+    const int v = vb + (int)threadIdx.x;
+    int n = 0, first = 0, m = 1, o0 = 0, tr = 0;
+    if (v < v1) {
+      int r = v, p = 0;
+      if (g.vrow) {  // the row holding virtual row v: largest r with vrow[r] <= v
+        int lo = 0, hi = g.rows;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (g.vrow[mid] <= v) lo = mid;
+          else hi = mid;
+        }
+        r = lo;
+        m = g.vrow[r + 1] - g.vrow[r];
+        p = v - g.vrow[r];
+      }
+      o0 = offsets[r];
+      const int L = offsets[r + 1] - o0;
+      auto lower = [&](int c) {  // first row-local index with column >= c
+        int lo = 0, hi = L;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (indices[o0 + mid] < c) lo = mid + 1;
+          else hi = mid;
+        }
+        return lo;
+      };
+      const int lo = lower(col_lo), hi = lower(col_hi);
+      // piece p of m holds the row's entries i with i % m == p
+      first = lo + ((p - lo % m) % m + m) % m;
+      n = first < hi ? (hi - 1 - first) / m + 1 : 0;
+      if (n > 0)
+        tr = g.excl ? g.excl[(long long)sg * g.V + v] - g.excl[(long long)sg * g.V + v0]
+                    : v - v0;
+    }
+    // exclusive scan of the run lengths over the workgroup
+    int inc = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int up = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += up;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    int carry = 0, total = 0;
+    for (int w = 0; w < kBlkBigTpb / 64; ++w) {
+      const int t = s_wave[w];
+      if (w < wave) carry += t;
+      total += t;
+    }
+    __syncthreads();  // s_wave is rewritten by the next tile
+    const long long at = pos + carry + inc - n;
+    for (int k = 0; k < n && at + k < end; ++k) {
+      const int nz = o0 + first + k * m;
+      packed[at + k] = (uint32_t)(indices[nz] - seg0);
+      trow[at + k] = (uint16_t)tr;
+      vals[at + k] = values[nz];
+    }
+    pos += total;
+  }
+  // Slots the runs did not reach (none when they add up to cnt[id]) hold a
+  // zero entry of the bucket's first column rather than leftover words.
+  for (long long i = pos + threadIdx.x; i < end; i += kBlkBigTpb) {
+    packed[i] = (uint32_t)(col_lo - seg0);
+    trow[i] = 0;
+    vals[i] = V{};
   }
 }
 
@@ -638,11 +755,18 @@ namespace {
 inline size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
 template <typename V>
-hipError_t blk_scatter(const csr_args& A, const blk_geom& g, int* cursor, uint16_t* trow,
-                       uint32_t* pk, void* vv, unsigned grid, hipStream_t s) {
+hipError_t blk_scatter(const csr_args& A, const blk_geom& g, const int* cnt, int* cursor,
+                       const int* big, int nbig, uint16_t* trow, uint32_t* pk, void* vv,
+                       unsigned grid, hipStream_t s) {
   blk_scatter_kernel<V><<<grid, 256, 0, s>>>(A.offsets, A.indices,
-                                             static_cast<const V*>(A.values), g, cursor, pk,
-                                             trow, static_cast<V*>(vv));
+                                             static_cast<const V*>(A.values), g, cnt, cursor,
+                                             pk, trow, static_cast<V*>(vv));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nbig == 0) return e;
+  // the bucket starts of the big buckets: their cursors, untouched above
+  blk_scatter_big_kernel<V><<<(unsigned)nbig, kBlkBigTpb, 0, s>>>(
+      A.offsets, A.indices, static_cast<const V*>(A.values), g, big, cnt, cursor, pk, trow,
+      static_cast<V*>(vv));
   return hipGetLastError();
 }
 
@@ -856,6 +980,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const size_t b_trow = al256(((size_t)A.nnz + 8) * sizeof(uint16_t));
   const size_t b_bstart = al256(((size_t)P.nb + 1) * sizeof(int));
   const size_t b_order = al256(((size_t)P.nb + 1) * sizeof(int));
+  const size_t b_big = al256(((size_t)A.nnz / (kBlkRound + 1) + 2) * sizeof(int));
   const size_t b_fl = comp ? al256((size_t)npairs * sizeof(int)) : 0;
   const size_t b_excl = comp ? al256((size_t)(npairs + 1) * sizeof(int)) : 0;
   // Entries in stream order (column | slot words, values) until they are
@@ -871,7 +996,8 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   };
   const size_t c_cnt = carve(b_cnt), c_start = carve(b_start), c_nr = carve(b_nr),
                c_first = carve(b_nr), c_scan = carve(b_scan), c_trow = carve(b_trow),
-               c_bstart = carve(b_bstart), c_fl = carve(b_fl), c_excl = carve(b_excl),
+               c_bstart = carve(b_bstart), c_big = carve(b_big), c_fl = carve(b_fl),
+               c_excl = carve(b_excl),
                c_work = carve(b_work), c_workv = carve(b_workv);
   (void)c_workv;
   char* sc = nullptr;
@@ -886,6 +1012,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   void* scan_scratch = sc + c_scan;
   uint16_t* trow = reinterpret_cast<uint16_t*>(sc + c_trow);
   int* bstart = reinterpret_cast<int*>(sc + c_bstart);
+  int* big = reinterpret_cast<int*>(sc + c_big);  // [0]: count, then the bucket ids
   int* fl = comp ? reinterpret_cast<int*>(sc + c_fl) : nullptr;
   int* excl = comp ? reinterpret_cast<int*>(sc + c_excl) : nullptr;
   char* work = sc + c_work;
@@ -918,6 +1045,16 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = launch_exclusive_scan(cnt, nbuckets, start, scan_scratch, s);
+  // buckets of more than a round, placed deterministically by the scatter
+  int nbig = 0;
+  if (e == hipSuccess) e = hipMemsetAsync(big, 0, sizeof(int), s);
+  if (e == hipSuccess && A.nnz > kBlkRound) {
+    long long lgrid = (nbuckets + 255) / 256;
+    if (lgrid > 65536) lgrid = 65536;
+    blk_list_big_kernel<<<(unsigned)lgrid, 256, 0, s>>>(cnt, nbuckets, big + 1, big);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&nbig, big, sizeof(int), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) {
     blk_round_count_kernel<<<sgrid, 256, 0, s>>>(start, (int)nstreams, nch, nr);
     e = hipGetLastError();
@@ -1023,13 +1160,13 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   if (e == hipSuccess && A.nnz > 0) {
     switch (A.vtype) {
       case LOOPS_VALUE_F32:
-        e = blk_scatter<float>(A, g, start, trow, pk, vv, (unsigned)grid, s);
+        e = blk_scatter<float>(A, g, cnt, start, big + 1, nbig, trow, pk, vv, (unsigned)grid, s);
         break;
       case LOOPS_VALUE_BF16:
-        e = blk_scatter<bf16_t>(A, g, start, trow, pk, vv, (unsigned)grid, s);
+        e = blk_scatter<bf16_t>(A, g, cnt, start, big + 1, nbig, trow, pk, vv, (unsigned)grid, s);
         break;
       case LOOPS_VALUE_F16:
-        e = blk_scatter<f16_t>(A, g, start, trow, pk, vv, (unsigned)grid, s);
+        e = blk_scatter<f16_t>(A, g, cnt, start, big + 1, nbig, trow, pk, vv, (unsigned)grid, s);
         break;
     }
   }

@@ -243,3 +243,28 @@ def test_blocks_shapes_of_the_auto_rule(rows, cols, nnz, zipf):
     torch.cuda.synchronize()
     _check(y.cpu().numpy(), B.offsets, B.indices, B.values, x_np)
     plan.destroy()
+
+
+@pytest.mark.parametrize("name", ["dense_narrow", "row_over_many_blocks", "mixed_lengths_8seg",
+                                  "powerlaw_60k"])
+def test_blocks_builds_are_identical(name):
+    """Two builds of a matrix give the same plan: buckets of more than a
+    round (dense_narrow's 64-column chunks, a long row's pieces) are placed
+    in (row, column) order before they are cut into rounds, so every row's
+    sum is grouped the same way; y is compared bit for bit over four
+    builds."""
+    off, idx, val, cols = MATS[name]
+    A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
+    x_np = np.random.default_rng(8).uniform(-1, 1, cols).astype(np.float32)
+    x = torch.from_numpy(x_np).cuda()
+    ys = []
+    for _ in range(4):
+        plan = loops.spmv_plan(A, layout="blocks")
+        y = torch.full((A.rows,), float("nan"), device="cuda")
+        plan.execute(x, y)
+        torch.cuda.synchronize()
+        ys.append(y.cpu().numpy())
+        plan.destroy()
+    for y in ys[1:]:
+        assert np.array_equal(y.view(np.uint32), ys[0].view(np.uint32))
+    _check(ys[0], off, idx, val, x_np)

@@ -115,6 +115,10 @@ class Server:
         out = torch.empty((rows,) if k == 1 else (rows, k), device="cuda")
         if out.numel():
             hip.memcpy(out.data_ptr(), yp, out.numel() * 4)
+            # a device-to-device hipMemcpy may return before the copy is
+            # done; the arena region must not be released (and handed to
+            # the next admitted request) while it is still being read
+            hip.stream_synchronize(None)
         self.release(ticket)
         return out
 

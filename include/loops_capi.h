@@ -396,7 +396,10 @@ loops_status_t loops_server_test(loops_server_t server, int64_t ticket, int32_t*
                                  float** y, double* submit_s, double* done_s);
 loops_status_t loops_server_wait(loops_server_t server, int64_t ticket, float** y,
                                  double* submit_s, double* done_s);
-/* Frees a completed request's output (arena memory for admission). */
+/* Frees a completed request's output (arena memory for admission): the
+ * next admitted request may write there at once, so every read of it must
+ * be complete first (a device-to-device hipMemcpy can return before it
+ * is: synchronise its stream). */
 loops_status_t loops_server_release(loops_server_t server, int64_t ticket);
 loops_status_t loops_server_now(loops_server_t server, double* now_s);
 loops_status_t loops_server_stats(loops_server_t server, loops_server_stats_t* stats);

@@ -45,6 +45,8 @@ SCHEDULES = {
 VALUE_F32 = 0
 VALUE_BF16 = 1
 VALUE_F16 = 2
+VALUE_F64 = 3
+VALUE_I32 = 4
 
 
 class CsrView(ctypes.Structure):
@@ -187,6 +189,11 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_size_t)]),
     "loops_spmv": (_st, [_vp, _st, ctypes.POINTER(CsrView), _vp, _vp,
                          ctypes.POINTER(Params), _vp, ctypes.c_size_t, _vp]),
+    "loops_spmv_typed_workspace_size": (_st, [_vp, _st, ctypes.POINTER(CsrView),
+                                              ctypes.POINTER(Params),
+                                              ctypes.POINTER(ctypes.c_size_t)]),
+    "loops_spmv_typed": (_st, [_vp, _st, ctypes.POINTER(CsrView), _vp, _vp,
+                               ctypes.POINTER(Params), _vp, ctypes.c_size_t, _vp]),
     "loops_spmv_noalloc": (_st, [_vp, _st, ctypes.POINTER(CsrView), _vp, _vp,
                                  ctypes.POINTER(Params), _vp]),
     "loops_merge_path_num_tiles": (_st, [_i64, _i64, _i64, ctypes.POINTER(_i64)]),

@@ -11,7 +11,8 @@ so each rejection can be exercised with host tensors.
 """
 from __future__ import annotations
 
-_VALUE_DTYPES = {"f32": "float32", "bf16": "bfloat16", "f16": "float16"}
+_VALUE_DTYPES = {"f32": "float32", "bf16": "bfloat16", "f16": "float16", "f64": "float64",
+                 "i32": "int32"}
 
 
 def _dtype_name(t):
@@ -65,10 +66,11 @@ def check_on(device, **tensors):
         _device_ok(t, device, name)
 
 
-def check_vector(t, n, name):
-    """fp32, 1-D, unit stride, at least n entries."""
-    if _dtype_name(t) != "float32":
-        raise ValueError(f"{name} must be float32, got {_dtype_name(t)}")
+def check_vector(t, n, name, dtype="float32"):
+    """1-D of the dtype (fp32 unless stated), unit stride, at least n
+    entries."""
+    if _dtype_name(t) != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {_dtype_name(t)}")
     if t.dim() != 1:
         raise ValueError(f"{name} must be 1-D")
     if t.numel() < n:

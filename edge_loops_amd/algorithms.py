@@ -45,10 +45,11 @@ def make_params(threads_per_block=0, items_per_thread=0, threads_per_tile=0,
 
 
 def workspace(schedule, csr, stream=None, **tuning):
-    """A device workspace sized for loops_spmv (contents need no
-    initialisation)."""
+    """A device workspace sized for loops_spmv (loops_spmv_typed for float64
+    / int32 matrices); contents need no initialisation."""
     import torch
-    n = max(spmv_workspace_size(schedule, csr, **tuning), 4)
+    size = spmv_typed_workspace_size if csr.value_type in ("f64", "i32") else spmv_workspace_size
+    n = max(size(schedule, csr, **tuning), 4)
     return torch.empty((n + 3) // 4 * 4, dtype=torch.uint8, device=csr.offsets.device)
 
 
@@ -64,7 +65,10 @@ def spmv_workspace_size(schedule, csr, **tuning):
 
 
 def run_spmv(schedule, csr, x, y, stream=None, workspace=None, **tuning):
-    """Asynchronous y = A x through loops_spmv (no synchronisation)."""
+    """Asynchronous y = A x through loops_spmv (no synchronisation); a
+    matrix of float64 or int32 values goes through loops_spmv_typed."""
+    if csr.value_type in ("f64", "i32"):
+        return run_spmv_typed(schedule, csr, x, y, stream, workspace, **tuning)
     sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
     _v.check_csr(csr)
     _v.check_vector(x, csr.cols, "x")
@@ -78,6 +82,38 @@ def run_spmv(schedule, csr, x, y, stream=None, workspace=None, **tuning):
     _capi.check(_capi.lib().loops_spmv(
         _capi.handle(), sched, ctypes.byref(v), _ptr(x), _ptr(y), ctypes.byref(p),
         ws, wsb, _stream(stream)), f"loops_spmv({schedule})")
+
+
+def spmv_typed_workspace_size(schedule, csr, **tuning):
+    sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
+    n = ctypes.c_size_t()
+    v = csr.view()
+    p = make_params(**tuning)
+    _capi.check(_capi.lib().loops_spmv_typed_workspace_size(
+        _capi.handle(), sched, ctypes.byref(v), ctypes.byref(p), ctypes.byref(n)),
+        "loops_spmv_typed_workspace_size")
+    return int(n.value)
+
+
+def run_spmv_typed(schedule, csr, x, y, stream=None, workspace=None, **tuning):
+    """Asynchronous y = A x through loops_spmv_typed: A's values, x and y
+    all float64 (value_type "f64") or all int32 ("i32", wrap-around)."""
+    sched = _capi.SCHEDULES[schedule] if isinstance(schedule, str) else schedule
+    if csr.value_type not in ("f64", "i32"):
+        raise ValueError("run_spmv_typed takes value_type 'f64' or 'i32' (use run_spmv)")
+    dt = "float64" if csr.value_type == "f64" else "int32"
+    _v.check_csr(csr)
+    _v.check_vector(x, csr.cols, "x", dt)
+    _v.check_vector(y, csr.rows, "y", dt)
+    _v.check_workspace(workspace)
+    _v.check_devices(csr, x=x, y=y, workspace=workspace)
+    v = csr.view()
+    p = make_params(**tuning)
+    ws = _ptr(workspace) if workspace is not None else None
+    wsb = workspace.numel() * workspace.element_size() if workspace is not None else 0
+    _capi.check(_capi.lib().loops_spmv_typed(
+        _capi.handle(), sched, ctypes.byref(v), _ptr(x), _ptr(y), ctypes.byref(p),
+        ws, wsb, _stream(stream)), f"loops_spmv_typed({schedule})")
 
 
 def spmm_workspace_size(schedule, csr, k, **tuning):

@@ -11,7 +11,8 @@ import numpy as np
 
 from . import _capi
 
-_VTYPES = {"f32": _capi.VALUE_F32, "bf16": _capi.VALUE_BF16, "f16": _capi.VALUE_F16}
+_VTYPES = {"f32": _capi.VALUE_F32, "bf16": _capi.VALUE_BF16, "f16": _capi.VALUE_F16,
+           "f64": _capi.VALUE_F64, "i32": _capi.VALUE_I32}
 
 
 def _torch():
@@ -22,7 +23,8 @@ def _torch():
 class csr_t:
     """CSR matrix on the device: rows, cols, nnzs and the arrays offsets
     (int32, rows+1), indices (int32, nnzs) and values (nnzs; fp32 by default,
-    or bf16 / fp16 storage with fp32 accumulation)."""
+    or bf16 / fp16 storage with fp32 accumulation, or float64 / int32 values
+    for the typed SpMV, run_spmv_typed)."""
 
     def __init__(self, rows, cols, offsets, indices, values, value_type="f32"):
         self.rows = int(rows)
@@ -43,6 +45,10 @@ class csr_t:
         torch = _torch()
         off = torch.from_numpy(np.ascontiguousarray(offsets, dtype=np.int32)).to(device)
         idx = torch.from_numpy(np.ascontiguousarray(indices, dtype=np.int32)).to(device)
+        if value_type in ("f64", "i32"):
+            dt = np.float64 if value_type == "f64" else np.int32
+            val = torch.from_numpy(np.ascontiguousarray(values, dtype=dt)).to(device)
+            return cls(rows, cols, off, idx, val, value_type)
         val = torch.from_numpy(np.ascontiguousarray(values, dtype=np.float32)).to(device)
         if value_type == "bf16":
             val = val.to(torch.bfloat16)

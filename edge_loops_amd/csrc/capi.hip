@@ -318,6 +318,104 @@ loops_status_t loops_spmv_noalloc(loops_handle_t handle, loops_schedule_t schedu
   return loops_spmv(handle, schedule, A, x, y, params, ws, r.ws_bytes, stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// check_csr for loops_spmv_typed: value types F64 / I32 only.
+loops_status_t check_typed_csr(const loops_csr_view_t* A, csr_args* out) {
+  if (!A) return LOOPS_ERROR_INVALID_VALUE;
+  if (A->value_type != LOOPS_VALUE_F64 && A->value_type != LOOPS_VALUE_I32)
+    return LOOPS_ERROR_INVALID_VALUE;
+  loops_csr_view_t v = *A;
+  v.value_type = LOOPS_VALUE_F32;  // the shared checks of sizes and pointers
+  const loops_status_t st = check_csr(&v, out);
+  out->vtype = A->value_type;
+  out->aligned = false;
+  return st;
+}
+
+loops_status_t resolve_typed(loops_handle_t h, loops_schedule_t s, const csr_args& A,
+                             const loops_params_t* p, typed_launch* L, long long* units) {
+  const loops_params_t zero{};
+  if (!p) p = &zero;
+  if (p->threads_per_block < 0 || p->items_per_thread < 0 || p->threads_per_tile < 0 ||
+      p->grid_size < 0)
+    return LOOPS_ERROR_INVALID_VALUE;
+  const bool wide = A.vtype == LOOPS_VALUE_F64;
+  *L = typed_launch{};
+  L->schedule = s;
+  *units = 0;
+  const long long total = (long long)A.rows + A.nnz;
+  switch (s) {
+    case LOOPS_SCHEDULE_MERGE_PATH_FLAT:
+      L->tpb = p->threads_per_block ? p->threads_per_block : (wide ? 64 : 128);
+      L->ipt = p->items_per_thread ? p->items_per_thread : (wide ? 3 : 5);
+      *units = (total + (long long)L->tpb * L->ipt - 1) / ((long long)L->tpb * L->ipt);
+      L->coords = p->tile_coordinates;
+      break;
+    case LOOPS_SCHEDULE_WORK_ORIENTED:
+      L->tpb = p->threads_per_block ? p->threads_per_block : kRefBlock;
+      L->grid = p->grid_size ? p->grid_size : 2 * (h ? h->cu_count : 256);
+      *units = total > 0 ? (long long)L->grid * L->tpb : 0;
+      break;
+    case LOOPS_SCHEDULE_GROUP_MAPPED:
+      L->tpb = p->threads_per_block ? p->threads_per_block : kRefBlock;
+      L->tpt = p->threads_per_tile ? p->threads_per_tile : L->tpb;
+      break;
+    case LOOPS_SCHEDULE_THREAD_MAPPED:
+    case LOOPS_SCHEDULE_ORIGINAL:
+      L->tpb = p->threads_per_block ? p->threads_per_block : kRefBlock;
+      break;
+    default:
+      return LOOPS_ERROR_NOT_SUPPORTED;
+  }
+  if (!typed_spmv_supported(s, L->tpb, L->ipt, L->tpt)) return LOOPS_ERROR_NOT_SUPPORTED;
+  L->ev0 = p->start_event;
+  L->ev1 = p->stop_event;
+  return LOOPS_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+loops_status_t loops_spmv_typed_workspace_size(loops_handle_t handle,
+                                               loops_schedule_t schedule,
+                                               const loops_csr_view_t* A,
+                                               const loops_params_t* params,
+                                               size_t* bytes) {
+  if (!bytes) return LOOPS_ERROR_INVALID_VALUE;
+  csr_args a;
+  loops_status_t st = check_typed_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  typed_launch L;
+  long long units = 0;
+  st = resolve_typed(handle, schedule, a, params, &L, &units);
+  if (st != LOOPS_SUCCESS) return st;
+  *bytes = units > 0 ? typed_fixup_bytes(units, a.vtype == LOOPS_VALUE_F64 ? 8 : 4) : 0;
+  return LOOPS_SUCCESS;
+}
+
+loops_status_t loops_spmv_typed(loops_handle_t handle, loops_schedule_t schedule,
+                                const loops_csr_view_t* A, const void* x, void* y,
+                                const loops_params_t* params, void* workspace,
+                                size_t workspace_bytes, hipStream_t stream) {
+  csr_args a;
+  loops_status_t st = check_typed_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  if (a.rows > 0 && !y) return LOOPS_ERROR_INVALID_VALUE;
+  if (a.nnz > 0 && !x) return LOOPS_ERROR_INVALID_VALUE;
+  typed_launch L;
+  long long units = 0;
+  st = resolve_typed(handle, schedule, a, params, &L, &units);
+  if (st != LOOPS_SUCCESS) return st;
+  const size_t need = units > 0 ? typed_fixup_bytes(units, a.vtype == LOOPS_VALUE_F64 ? 8 : 4) : 0;
+  st = get_workspace(handle, need, workspace, workspace_bytes, &L.ws);
+  if (st != LOOPS_SUCCESS) return st;
+  return hip_status(launch_typed_spmv(L, a, x, y, stream));
+}
+
 loops_status_t loops_merge_path_num_tiles(int64_t rows, int64_t nnzs,
                                           int64_t items_per_tile,
                                           int64_t* num_tiles) {

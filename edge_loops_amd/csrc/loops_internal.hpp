@@ -40,6 +40,19 @@ inline size_t fixup_ws_bytes(long long units) {
 }
 fixup_ws carve_fixup_ws(void* ws, long long units);
 
+// ---- SpMV with double or int32 values, x and y (spmv_typed.hip) ----------
+struct typed_launch {
+  loops_schedule_t schedule;
+  int tpb, ipt, tpt, grid;
+  const loops_coord_t* coords;  // merge_path_flat tile coordinates, or null
+  void* ws;                     // fix-up slots (merge_path_flat, work_oriented)
+  hipEvent_t ev0, ev1;
+};
+bool typed_spmv_supported(loops_schedule_t schedule, int tpb, int ipt, int tpt);
+size_t typed_fixup_bytes(long long units, int value_bytes);
+hipError_t launch_typed_spmv(const typed_launch& L, const csr_args& A, const void* x, void* y,
+                             hipStream_t s);
+
 // ---- SpMV ----------------------------------------------------------------
 bool mp_spmv_supported(int tpb, int ipt);
 hipError_t launch_mp_spmv(const csr_args& A, const float* x, float* y, int tpb,

@@ -43,15 +43,26 @@ inline loops_handle_t handle() {
   return t.h;
 }
 
+/// The value type of type_t: float (loops_spmv), double or int
+/// (loops_spmv_typed, the reference's other instantiations,
+/// benchmarks/spmv/work_oriented.cu:53).
+template <typename type_t>
+constexpr loops_value_type_t value_type_of() {
+  static_assert(std::is_same<type_t, float>::value || std::is_same<type_t, double>::value ||
+                    (std::is_same<type_t, int>::value && sizeof(int) == 4),
+                "the C++ entry points take float, double or int values (bf16 / fp16 "
+                "storage is available through the C ABI's value_type)");
+  return std::is_same<type_t, float>::value    ? LOOPS_VALUE_F32
+         : std::is_same<type_t, double>::value ? LOOPS_VALUE_F64
+                                               : LOOPS_VALUE_I32;
+}
+
 template <typename index_t, typename offset_t, typename type_t>
 loops_csr_view_t view(const csr_t<index_t, offset_t, type_t, memory_space_t::device>& csr) {
   static_assert(std::is_integral<index_t>::value && sizeof(index_t) == 4,
                 "the library takes 32-bit column indices");
   static_assert(std::is_integral<offset_t>::value && sizeof(offset_t) == 4,
                 "the library takes 32-bit row offsets");
-  static_assert(std::is_same<type_t, float>::value,
-                "the C++ entry points take fp32 values (bf16 / fp16 storage is "
-                "available through the C ABI's value_type)");
   loops_csr_view_t v;
   v.rows = (int64_t)csr.rows;
   v.cols = (int64_t)csr.cols;
@@ -59,7 +70,7 @@ loops_csr_view_t view(const csr_t<index_t, offset_t, type_t, memory_space_t::dev
   v.offsets = reinterpret_cast<const int32_t*>(csr.offsets.raw());
   v.indices = reinterpret_cast<const int32_t*>(csr.indices.raw());
   v.values = csr.values.raw();
-  v.value_type = LOOPS_VALUE_F32;
+  v.value_type = value_type_of<type_t>();
   return v;
 }
 
@@ -77,8 +88,12 @@ void spmv(loops_schedule_t schedule,
   loops_params_t p{};
   p.start_event = ev0;
   p.stop_event = ev1;
-  check(loops_spmv(handle(), schedule, &v, x.raw(), y.raw(), &p, nullptr, 0, stream),
-        loops_schedule_name(schedule));
+  if constexpr (std::is_same<type_t, float>::value)
+    check(loops_spmv(handle(), schedule, &v, x.raw(), y.raw(), &p, nullptr, 0, stream),
+          loops_schedule_name(schedule));
+  else
+    check(loops_spmv_typed(handle(), schedule, &v, x.raw(), y.raw(), &p, nullptr, 0, stream),
+          loops_schedule_name(schedule));
   error::throw_if_exception(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
@@ -88,6 +103,7 @@ void spmm(loops_schedule_t schedule,
           matrix_t_& B, matrix_t_& C, hipStream_t stream) {
   error::throw_if_exception(B.rows < csr.cols, "B has fewer rows than A has columns");
   error::throw_if_exception(C.rows < csr.rows || C.cols < B.cols, "C too small");
+  static_assert(std::is_same<type_t, float>::value, "SpMM takes fp32 values");
   loops_csr_view_t v = view(csr);
   loops_params_t p{};
   check(loops_spmm(handle(), schedule, &v, (int64_t)B.cols, B.m_data_ptr,

@@ -65,11 +65,16 @@ typedef enum loops_schedule_t {
   LOOPS_SCHEDULE_ORIGINAL = 5
 } loops_schedule_t;
 
-/* Value storage of A.  Accumulation is always fp32; x, y, B and C are fp32. */
+/* Value storage of A.  For every entry point but loops_spmv_typed:
+ * F32 / BF16 / F16 storage, fp32 accumulation, fp32 x, y, B and C.
+ * loops_spmv_typed takes F64 or I32: A's values, x and y all of that type
+ * and every product and sum in it (I32: 32-bit wrap-around). */
 typedef enum loops_value_type_t {
   LOOPS_VALUE_F32 = 0,
   LOOPS_VALUE_BF16 = 1,
-  LOOPS_VALUE_F16 = 2
+  LOOPS_VALUE_F16 = 2,
+  LOOPS_VALUE_F64 = 3,
+  LOOPS_VALUE_I32 = 4
 } loops_value_type_t;
 
 /* container/csr.hxx:36-43 as a non-owning view (device pointers). */
@@ -162,6 +167,26 @@ loops_status_t loops_spmv(loops_handle_t handle, loops_schedule_t schedule,
 loops_status_t loops_spmv_noalloc(loops_handle_t handle, loops_schedule_t schedule,
                                   const loops_csr_view_t* A, const float* x, float* y,
                                   const loops_params_t* params, hipStream_t stream);
+
+/* y = A * x with A->value_type LOOPS_VALUE_F64 (x, y double) or
+ * LOOPS_VALUE_I32 (x, y int32): the reference's host entries for
+ * type_t = double and int (algorithms/spmv/*.cuh; its benchmark's value
+ * types, benchmarks/spmv/work_oriented.cu:53).  Same schedules, partitions
+ * and contract as loops_spmv (y overwritten, fixed-order sums); default
+ * tiles follow the reference per value size: merge_path_flat 64 x 3 for
+ * double, 128 x 5 for int32 (merge_path_flat.cuh:102-103); also 256 x 4
+ * and 256 x 8.  work_oriented: 64, 128 or 256 threads; group_mapped: the
+ * loops_spmv tunings (threads_per_tile 32 .. TPB).  The workspace rule is
+ * loops_spmv's (loops_spmv_typed_workspace_size). */
+loops_status_t loops_spmv_typed_workspace_size(loops_handle_t handle,
+                                               loops_schedule_t schedule,
+                                               const loops_csr_view_t* A,
+                                               const loops_params_t* params,
+                                               size_t* bytes);
+loops_status_t loops_spmv_typed(loops_handle_t handle, loops_schedule_t schedule,
+                                const loops_csr_view_t* A, const void* x, void* y,
+                                const loops_params_t* params, void* workspace,
+                                size_t workspace_bytes, hipStream_t stream);
 
 /* Number of merge tiles, ceil((rows + nnzs) / items_per_tile)
  * (merge_path_flat.cuh:116-117). */

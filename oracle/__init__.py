@@ -44,6 +44,12 @@ def lib():
         L.oracle_spmv_f32.restype = None
         L.oracle_spmv_f64.argtypes = [_i64, _i32p, _i32p, _f32p, _f32p, _f64p, _f64p]
         L.oracle_spmv_f64.restype = None
+        L.oracle_spmv_d.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.oracle_spmv_d.restype = None
+        L.oracle_spmv_d_ref.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
+        L.oracle_spmv_d_ref.restype = None
+        L.oracle_spmv_i32.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _i32p]
+        L.oracle_spmv_i32.restype = None
         L.oracle_spmv_f32_mt.argtypes = [ctypes.c_int, _i64, _i32p, _i32p, _f32p, _f32p, _f32p]
         L.oracle_spmv_f32_mt.restype = ctypes.c_int
         L.oracle_count_abs_errors.argtypes = [_i64, _f32p, _f32p, ctypes.c_double]
@@ -106,6 +112,34 @@ def spmv_f32_mt(offsets, indices, values, x, nthreads, out=None):
         raise ValueError("out must be a contiguous float32 array of rows entries")
     if lib().oracle_spmv_f32_mt(int(nthreads), rows, offsets, indices, values, x, y) != 0:
         raise RuntimeError("oracle_spmv_f32_mt: thread creation failed")
+    return y
+
+
+def spmv_d(offsets, indices, values, x):
+    """The validator loop for type_t = double (CSR order, double)."""
+    offsets, indices = _c(offsets, np.int32), _c(indices, np.int32)
+    values, x = _c(values, np.float64), _c(x, np.float64)
+    y = np.empty(offsets.shape[0] - 1, np.float64)
+    lib().oracle_spmv_d(y.shape[0], offsets, indices, values, x, y)
+    return y
+
+
+def spmv_d_ref(offsets, indices, values, x):
+    """Long-double reference for double SpMV: (y, sum_j |a_ij x_j|)."""
+    offsets, indices = _c(offsets, np.int32), _c(indices, np.int32)
+    values, x = _c(values, np.float64), _c(x, np.float64)
+    y = np.empty(offsets.shape[0] - 1, np.float64)
+    a = np.empty_like(y)
+    lib().oracle_spmv_d_ref(y.shape[0], offsets, indices, values, x, y, a)
+    return y, a
+
+
+def spmv_i32(offsets, indices, values, x):
+    """The validator loop for type_t = int (32-bit wrap-around)."""
+    offsets, indices = _c(offsets, np.int32), _c(indices, np.int32)
+    values, x = _c(values, np.int32), _c(x, np.int32)
+    y = np.empty(offsets.shape[0] - 1, np.int32)
+    lib().oracle_spmv_i32(y.shape[0], offsets, indices, values, x, y)
     return y
 
 

@@ -28,6 +28,12 @@
  *   oracle_mp_thread_coords  the sub-path property vs the brute-force walk.
  *   oracle_wo_coords       == search at clamped tid*wpt, brute force.
  *   oracle_gm_atom_row     brute force over the exclusive scan.
+ *   oracle_spmv_d          integer known answers (exact in double), the
+ *                          sequential double loop element by element, and
+ *                          a cancellation the long-double reference keeps.
+ *   oracle_spmv_d_ref      2^54 + 1 - 2^54 = 1 (double: 0), |y| <= abs.
+ *   oracle_spmv_i32        wrap-around known answers (2^31 products), and
+ *                          the result is the same for any summation order.
  *   oracle_spmm_f32        column-by-column equality with oracle_spmv_f32.
  *   oracle_spmm_f64        the same fp64 known answers per column, and
  *                          column-by-column equality with oracle_spmv_f64.
@@ -74,6 +80,50 @@ void oracle_spmv_f64(int64_t rows, const int32_t* offsets,
     }
     y[row] = s;
     if (abs_sum) abs_sum[row] = a;
+  }
+}
+
+/* The validator loop for type_t = double (the reference's host entries are
+ * instantiated for double too: benchmarks/spmv/work_oriented.cu:53):
+ * double products and sums in CSR order. */
+void oracle_spmv_d(int64_t rows, const int32_t* offsets, const int32_t* indices,
+                   const double* values, const double* x, double* y) {
+  for (int64_t row = 0; row < rows; ++row) {
+    double sum = 0.0;
+    for (int32_t nz = offsets[row]; nz < offsets[row + 1]; ++nz) {
+      double prod = values[nz] * x[indices[nz]];
+      sum = sum + prod;
+    }
+    y[row] = sum;
+  }
+}
+
+/* A reference for double results: long double (x86 80-bit, 64-bit
+ * mantissa) products and sums, and the per-row sum_j |a_ij x_j|. */
+void oracle_spmv_d_ref(int64_t rows, const int32_t* offsets, const int32_t* indices,
+                       const double* values, const double* x, double* y, double* abs_sum) {
+  for (int64_t row = 0; row < rows; ++row) {
+    long double s = 0.0L, a = 0.0L;
+    for (int32_t nz = offsets[row]; nz < offsets[row + 1]; ++nz) {
+      long double p = (long double)values[nz] * (long double)x[indices[nz]];
+      s += p;
+      a += fabsl(p);
+    }
+    y[row] = (double)s;
+    abs_sum[row] = (double)a;
+  }
+}
+
+/* The validator loop for type_t = int: products and sums wrap around
+ * modulo 2^32 (unsigned arithmetic, so no overflow is undefined), which
+ * makes every summation order give the same bits. */
+void oracle_spmv_i32(int64_t rows, const int32_t* offsets, const int32_t* indices,
+                     const int32_t* values, const int32_t* x, int32_t* y) {
+  for (int64_t row = 0; row < rows; ++row) {
+    uint32_t sum = 0;
+    for (int32_t nz = offsets[row]; nz < offsets[row + 1]; ++nz)
+      sum += (uint32_t)values[nz] * (uint32_t)x[indices[nz]];
+    y[row] = (int32_t)sum;
   }
 }
 

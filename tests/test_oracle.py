@@ -376,3 +376,66 @@ def test_mtx_symmetric_expansion_order(tmp_path):
     assert off.tolist() == [0, 3, 5, 7]
     assert idx.tolist() == [1, 1, 2, 0, 0, 0, 2]
     assert val.tolist() == [1.0, 3.0, 2.0, 1.0, 3.0, 2.0, 4.0]
+
+
+# ---------------------------------------------------------------------------
+# double and int value types (the reference's other instantiations)
+# ---------------------------------------------------------------------------
+def test_spmv_d_known_answers_and_order():
+    off, idx, _ = random_csr(200, 150, 30, seed=51)
+    rng = np.random.default_rng(52)
+    val = rng.integers(-8, 9, idx.shape[0]).astype(np.float64)
+    x = rng.integers(-8, 9, 150).astype(np.float64)
+    dense = np.zeros((200, 150))
+    for r in range(200):
+        dense[r, idx[off[r]:off[r + 1]]] = val[off[r]:off[r + 1]]
+    assert np.array_equal(oracle.spmv_d(off, idx, val, x), dense @ x)
+    # random values: the left-to-right double sum, element by element
+    val = rng.uniform(-1, 1, idx.shape[0])
+    x = rng.uniform(-1, 1, 150)
+    y = oracle.spmv_d(off, idx, val, x)
+    for r in range(0, 200, 7):
+        s = 0.0
+        for nz in range(off[r], off[r + 1]):
+            s = s + val[nz] * x[idx[nz]]
+        assert y[r] == s
+
+
+def test_spmv_d_ref_keeps_what_double_loses():
+    # 2^54 + 1 - 2^54 with x = 1: exact 1 (long double has a 64-bit
+    # mantissa); a double accumulator gives 0 (2^54 + 1 rounds to 2^54)
+    off = np.array([0, 3, 3], np.int32)
+    idx = np.array([0, 1, 2], np.int32)
+    val = np.array([2.0 ** 54, 1.0, -(2.0 ** 54)])
+    x = np.ones(3)
+    y, a = oracle.spmv_d_ref(off, idx, val, x)
+    assert y.tolist() == [1.0, 0.0] and a.tolist() == [2.0 ** 55 + 1, 0.0]
+    assert oracle.spmv_d(off, idx, val, x).tolist() == [0.0, 0.0]
+    off2, idx2, _ = random_csr(100, 300, 200, seed=53)
+    v2 = np.random.default_rng(54).uniform(-1, 1, idx2.shape[0])
+    x2 = np.random.default_rng(55).uniform(-1, 1, 300)
+    yr, ar = oracle.spmv_d_ref(off2, idx2, v2, x2)
+    assert np.all(np.abs(yr) <= ar)
+    n = np.diff(off2).astype(np.float64)
+    assert np.all(np.abs(oracle.spmv_d(off2, idx2, v2, x2) - yr) <= (n + 1) * 2.0 ** -53 * ar)
+
+
+def test_spmv_i32_wraps_and_ignores_order():
+    off = np.array([0, 2, 5], np.int32)
+    idx = np.array([0, 1, 0, 1, 2], np.int32)
+    val = np.array([2 ** 16, 2 ** 15, -7, 3, 2 ** 30], np.int32)
+    x = np.array([2 ** 15, 2, 4], np.int32)
+    y = oracle.spmv_i32(off, idx, val, x)
+    # row 0: 2^31 + 2^16 wraps to -2^31 + 2^16; row 1: -7*2^15 + 6 + 2^32 -> wraps
+    assert y.tolist() == [np.int32(-(2 ** 31) + 2 ** 16), np.int32(-7 * 2 ** 15 + 6)]
+    off2, idx2, _ = random_csr(300, 400, 60, seed=56)
+    rng = np.random.default_rng(57)
+    v2 = rng.integers(-2 ** 31, 2 ** 31, idx2.shape[0], dtype=np.int64).astype(np.int32)
+    x2 = rng.integers(-2 ** 31, 2 ** 31, 400, dtype=np.int64).astype(np.int32)
+    y2 = oracle.spmv_i32(off2, idx2, v2, x2)
+    # any order: numpy's wrapping int64 products summed per row, mod 2^32
+    ref = np.zeros(300, np.int64)
+    for r in range(300):
+        p = v2[off2[r]:off2[r + 1]].astype(np.int64) * x2[idx2[off2[r]:off2[r + 1]]].astype(np.int64)
+        ref[r] = int(p[::-1].sum()) & 0xffffffff
+    assert np.array_equal(y2.view(np.uint32).astype(np.int64), ref)

@@ -86,6 +86,7 @@ class PlanParams(ctypes.Structure):
         ("items_per_thread", ctypes.c_int32),
         ("panels", ctypes.c_int32),
         ("layout", ctypes.c_int32),
+        ("for_batches", ctypes.c_int32),
     ]
 
 
@@ -99,6 +100,7 @@ class BatchItem(ctypes.Structure):
         ("ldy", ctypes.c_int64),
         ("schedule", ctypes.c_int),
         ("items_per_thread", ctypes.c_int32),
+        ("plan", ctypes.c_void_p),
     ]
 
 
@@ -114,6 +116,8 @@ class BatchInfo(ctypes.Structure):
         ("max_items_per_thread", ctypes.c_int64),
         ("spmm_variant_mask", ctypes.c_int64),
         ("spmv_class_mask", ctypes.c_int64),
+        ("blocks_work_items", ctypes.c_int64),
+        ("blocks_row_pieces", ctypes.c_int64),
     ]
 
 

@@ -220,10 +220,14 @@ class spmv_plan:
     x read in column order; "auto": blocks for matrices with at least 2^12
     columns and 2^19 nonzeros when no panels / threads_per_block /
     items_per_thread is given, else panels.  Both are deterministic.  ``execute(x, y)`` is
-    asynchronous and allocates nothing."""
+    asynchronous and allocates nothing.  ``for_batches=True`` cuts a
+    blocks plan for batch items (edge_loops_amd.batch): one column
+    segment when cols <= 2^20 and the fewest row blocks the LDS holds, so
+    each workgroup streams many full rounds; alone such a plan underfills
+    the GPU."""
 
     def __init__(self, csr, panels=0, threads_per_block=0, items_per_thread=0,
-                 stream=None, layout="auto"):
+                 stream=None, layout="auto", for_batches=False):
         _v.check_csr(csr)
         _v.check_devices(csr)
         self.csr = csr  # keeps A alive: with one panel the plan reads it in place
@@ -231,7 +235,7 @@ class spmv_plan:
         self.device = csr.offsets.device
         v = csr.view()
         p = _capi.PlanParams(int(threads_per_block), int(items_per_thread), int(panels),
-                             _capi.PLAN_LAYOUTS[layout])
+                             _capi.PLAN_LAYOUTS[layout], int(bool(for_batches)))
         h = ctypes.c_void_p()
         _capi.check(_capi.lib().loops_spmv_plan_create(
             _capi.handle(), ctypes.byref(v), ctypes.byref(p), _stream(stream),

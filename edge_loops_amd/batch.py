@@ -8,6 +8,13 @@
 
 x / y are 1-D fp32 device tensors (SpMV) or 2-D row-major (SpMM, k =
 columns).  An item's result does not depend on the rest of the batch.
+
+An SpMV item may carry a row-block plan of its matrix
+(``BatchItem(A, x, y, "blocks", plan=spmv_plan(A, layout="blocks"))``):
+it then runs the plan's streams and gives what ``plan.execute`` gives, bit
+for bit.  A plan may appear in one item of a batch only, and a batch's
+launches must not overlap other executions of its plans (they share the
+plan's partial sums).
 """
 from __future__ import annotations
 
@@ -18,10 +25,13 @@ from . import _validate as _v
 
 
 class BatchItem:
-    def __init__(self, csr, x, y, schedule="merge_path_flat", items_per_thread=0):
+    def __init__(self, csr, x, y, schedule="merge_path_flat", items_per_thread=0, plan=None):
+        if schedule == "blocks" and plan is None:
+            raise ValueError("a 'blocks' batch item needs a row-block plan (plan=)")
         self.csr, self.x, self.y = csr, x, y
         self.schedule = schedule
         self.items_per_thread = int(items_per_thread)
+        self.plan = plan  # kept alive with the item
 
     def to_c(self):
         _v.check_csr(self.csr)
@@ -30,11 +40,25 @@ class BatchItem:
         k = 1 if self.x.dim() == 1 else int(self.x.shape[1])
         ldx = 1 if self.x.dim() == 1 else int(self.x.stride(0))
         ldy = 1 if self.y.dim() == 1 else int(self.y.stride(0))
-        sched = _capi.SCHEDULES[self.schedule] if isinstance(self.schedule, str) else self.schedule
+        plan = None
+        if self.plan is not None:
+            if self.plan.layout != "blocks":
+                raise ValueError(f"batch items take row-block plans, not {self.plan.layout!r}")
+            if k != 1:
+                raise ValueError("a plan item is an SpMV (1-D x and y)")
+            if (self.plan.rows, self.plan.cols) != (self.csr.rows, self.csr.cols):
+                raise ValueError("the plan was built for a matrix of another shape")
+            if not getattr(self.plan, "_h", None):
+                raise ValueError("the plan has been destroyed")
+            plan = self.plan._h.value
+        sched = self.schedule
+        if sched == "blocks":
+            sched = "merge_path_flat"  # not read for plan items
+        sched = _capi.SCHEDULES[sched] if isinstance(sched, str) else sched
         return _capi.BatchItem(self.csr.view(), k,
                                self.x.data_ptr() if self.x.numel() else None, ldx,
                                self.y.data_ptr() if self.y.numel() else None, ldy,
-                               sched, self.items_per_thread)
+                               sched, self.items_per_thread, plan)
 
 
 def query(items):

@@ -14,7 +14,10 @@
 //          of moderate, even length;
 //   SPMM   one merge-path SpMM tile of an SpMM item (the single-matrix SpMM
 //          tile: ranges of G lanes spanning its columns, 1 or 4 columns per
-//          lane, IPT = 4, 8 or 16 steps per range).
+//          lane, IPT = 4, 8 or 16 steps per range);
+//   BLOCKS one (block, segment) stream of an item carrying a row-block
+//          plan (the plan's own stream code, 512 lanes, y of the block in
+//          LDS); a combine kernel then adds each row's segment partials.
 // The pieces of all items form one concatenated range per kernel; a piece
 // finds its item by a search over the prefix counts (a 64-lane cooperative
 // search, done again only when a workgroup's next piece leaves the item of
@@ -32,6 +35,7 @@
 // the batch.  Launching allocates nothing and can be captured in a hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include <cstdint>
@@ -39,6 +43,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "blk_stream.hpp"
 #include "device_common.hpp"
 #include "loops_internal.hpp"
 #include "merge_tile.hpp"
@@ -46,17 +51,27 @@
 
 namespace loops_mi355x {
 
-enum batch_kind : int { BK_MERGE = 0, BK_ROWS = 1, BK_SPMM = 2, BK_GROUP = 3 };
+enum batch_kind : int { BK_MERGE = 0, BK_ROWS = 1, BK_SPMM = 2, BK_GROUP = 3, BK_BLOCKS = 4 };
 
 // Kernel classes, one persistent kernel each, so that every kernel has the
 // registers, LDS and occupancy of the single-matrix kernel for its tile:
 //   SpMV classes 0..2: merge tiles of 4 (and the ROWS / GROUP pieces), 8 and
 //                      16 steps per lane;
 //   SpMM variants:     value type x columns per lane (1, 4) x merge steps
-//                      per range (4, 8, 16), class kSpmvClasses + variant.
+//                      per range (4, 8, 16), class kSpmvClasses + variant;
+//   row-block streams: class kBlkClass.
 constexpr int kSpmvClasses = 3;
 constexpr int kSpmmVariants = 3 * 2 * 3;
-constexpr int kClasses = kSpmvClasses + kSpmmVariants;
+constexpr int kBlkClass = kSpmvClasses + kSpmmVariants;
+constexpr int kClasses = kBlkClass + 1;
+// A plan item of several segments has its streams padded to a multiple of
+// this many pieces, and such items come first in their class, so that
+// their pieces start at multiples of 8 within the class: with 8 queue
+// subsets (piece i in subset i % 8, taken by workgroups blockIdx.x % 8 ==
+// i % 8, and workgroups go to the 8 XCDs round-robin) segment s of every
+// plan then runs on the XCDs s % 8, as in the plan's own launch, and each
+// XCD's L2 holds only its slices of x.  One-segment items are not padded.
+constexpr int kBlkPiecePad = 8;
 __host__ __device__ constexpr int spmm_variant(int vtype, int cpl, int ipt) {
   return (vtype * 2 + (cpl == 4 ? 1 : 0)) * 3 + (ipt == 4 ? 0 : ipt == 8 ? 1 : 2);
 }
@@ -78,7 +93,8 @@ struct batch_item_dev {
   int vtype, kind, ipt, g;  // ipt: merge steps per lane (MERGE) / range (SPMM)
                             // g: lanes per row (GROUP) or per range (SPMM)
   int units, first_slot, cpl, first_coord;  // units: SPMM tiles; cpl: columns per lane
-  int ti, pad0, pad1, pad2;                 // ti: merge steps per tile (MERGE, SPMM)
+  int ti, blk, pad1, pad2;  // ti: merge steps per tile (MERGE, SPMM); blk: BLOCKS
+                            // item's entry in the table's plan arguments
 };
 
 // Prefix arrays of the table, n_items + 1 entries each: entry i = pieces /
@@ -87,7 +103,8 @@ enum batch_prefix : int {
   BP_TILE = 0,   // merge-tile fix-up slots (MERGE items)
   BP_COORD = 1,  // merge-tile start coordinates (MERGE and SPMM items)
   BP_UNIT = 2,   // SpMM tile fix-up slots (SPMM items)
-  BP_COUNT = 3
+  BP_BROWS = 3,  // 256-row combine pieces (BLOCKS items)
+  BP_COUNT = 4
 };
 
 // Items are also listed grouped by kernel class (order, stable in item
@@ -95,7 +112,8 @@ enum batch_prefix : int {
 // order, and cprefix[j] counts the pieces of entries before j.
 struct batch_header {
   long long n_items, n_spmv_work, n_spmm_work, n_tiles, n_coords, n_units;
-  long long items_off, prefix_off, order_off, cprefix_off;
+  long long n_blk_work, n_brows, n_blk;
+  long long items_off, prefix_off, order_off, cprefix_off, blk_off;
   long long class_begin[kClasses + 1];
   unsigned long long* queue;  // work-queue counter sets, one per class
 };
@@ -345,6 +363,43 @@ __global__ __launch_bounds__(kBatchTpb) void batch_spmm_kernel(const char* __res
   });
 }
 
+// Row-block streams of the plan items: piece sub of an item is stream sub
+// of its plan (padding pieces past nb * S do nothing).  The LDS and the
+// stream code are the plan's own launch's.
+__global__ __launch_bounds__(kBlkTpb) void batch_blocks_kernel(const char* __restrict__ table) {
+  __shared__ blk_smem sm;
+  __shared__ long long s_next;
+  const batch_header* h = reinterpret_cast<const batch_header*>(table);
+  const batch_item_dev* items =
+      reinterpret_cast<const batch_item_dev*>(table + h->items_off);
+  const blk_exec_args* blk = reinterpret_cast<const blk_exec_args*>(table + h->blk_off);
+  batch_class_loop(table, kBlkClass, &s_next, [&](int item, int sub) {
+    const batch_item_dev& it = items[item];
+    const blk_exec_args& a = blk[it.blk];
+    if (sub >= a.nb * a.S) return;
+    if (it.vtype == LOOPS_VALUE_BF16) blk_stream<bf16_t>(a, it.x, sub, sm);
+    else if (it.vtype == LOOPS_VALUE_F16) blk_stream<f16_t>(a, it.x, sub, sm);
+    else blk_stream<float>(a, it.x, sub, sm);
+  });
+}
+
+// The plan items' combine: workgroup p takes 256 rows of the item found in
+// the BP_BROWS prefix counts and adds each row's segment partials in the
+// plan's order.
+__global__ __launch_bounds__(256) void batch_blocks_combine_kernel(const char* __restrict__ table) {
+  const batch_header* h = reinterpret_cast<const batch_header*>(table);
+  const batch_item_dev* items =
+      reinterpret_cast<const batch_item_dev*>(table + h->items_off);
+  const blk_exec_args* blk = reinterpret_cast<const blk_exec_args*>(table + h->blk_off);
+  const long long* P = batch_prefix_of(table, h, BP_BROWS);
+  const long long p = blockIdx.x;
+  const int item = batch_wave_find_item(P, h->n_items, p);
+  const batch_item_dev& it = items[item];
+  const blk_exec_args& a = blk[it.blk];
+  const long long r = (p - P[item]) * 256 + threadIdx.x;
+  if (r < a.rows) it.y[r] = blk_row_sum(a.partial, a.part_stride, a.S, a.vrow, (int)r);
+}
+
 // Merge-tile fix-up across the batch: global slot s belongs to the item
 // found in the BP_TILE prefix counts.
 __global__ __launch_bounds__(256) void batch_spmv_fixup_kernel(const char* __restrict__ table) {
@@ -455,10 +510,32 @@ namespace {
 struct item_counts {
   int kind = BK_MERGE, ipt = 0, g = 0, cpl = 0, ti = 0, cls = 0;
   long long spmv_work = 0, spmm_work = 0, tiles = 0, coords = 0, units = 0;
+  long long blk_work = 0, brows = 0;
   size_t spmm_ws = 0;
 };
 
+// A plan item: A's rows and cols must be the plan's; its pointers are not
+// used.
+loops_status_t count_plan_item(const loops_batch_item_t& b, item_counts* c) {
+  const loops_spmv_plan* pl = b.plan;
+  if (pl->st.layout != LOOPS_PLAN_LAYOUT_BLOCKS) return LOOPS_ERROR_NOT_SUPPORTED;
+  if (b.k != 1 || b.A.rows != pl->rows || b.A.cols != pl->cols) return LOOPS_ERROR_INVALID_VALUE;
+  const blk_plan_state& st = pl->st.blk;
+  if ((st.rows > 0 && !b.y) || (st.nnz > 0 && !b.x)) return LOOPS_ERROR_INVALID_VALUE;
+  *c = item_counts{};
+  c->kind = BK_BLOCKS;
+  c->cls = kBlkClass;
+  if (st.rows == 0) return LOOPS_SUCCESS;  // the plan holds nothing to run
+  const long long n = (long long)st.nb * st.S;
+  // one segment: every XCD reads all of x, so nothing to align
+  const long long pad = st.S > 1 ? kBlkPiecePad : 1;
+  c->blk_work = (n + pad - 1) / pad * pad;
+  c->brows = st.direct ? 0 : ((long long)st.rows + 255) / 256;  // direct: the streams write y
+  return LOOPS_SUCCESS;
+}
+
 loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
+  if (b.plan) return count_plan_item(b, c);
   const loops_csr_view_t& A = b.A;
   if (A.rows < 0 || A.cols < 0 || A.nnzs < 0 || b.k < 1 || b.k > (1 << 20))
     return LOOPS_ERROR_INVALID_VALUE;
@@ -571,6 +648,7 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
                                   loops_batch_info_t* info, batch_item_dev* out_table,
                                   char* dev_ws) {
   long long n_spmv_work = 0, n_spmm_work = 0, n_tiles = 0, n_coords = 0, n_units = 0;
+  long long n_blk_work = 0, n_brows = 0, n_blk = 0;
   int max_ipt = 0;
   size_t spmm_ws_bytes = 0;
   long long cls_count[kClasses] = {};
@@ -588,12 +666,24 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
     n_tiles += c.tiles;
     n_coords += c.coords;
     n_units += c.units;
+    n_blk_work += c.blk_work;
+    n_brows += c.brows;
+    if (c.kind == BK_BLOCKS) ++n_blk;
     spmm_ws_bytes += c.spmm_ws;
     if (c.kind == BK_MERGE && c.ipt > max_ipt) max_ipt = c.ipt;
   }
+  if (n_blk > 1) {  // the streams of one plan share its partial sums
+    std::vector<const loops_spmv_plan*> plans;
+    plans.reserve((size_t)n_blk);
+    for (int64_t i = 0; i < n; ++i)
+      if (items[i].plan) plans.push_back(items[i].plan);
+    std::sort(plans.begin(), plans.end());
+    if (std::adjacent_find(plans.begin(), plans.end()) != plans.end())
+      return LOOPS_ERROR_INVALID_VALUE;
+  }
   if (n_spmv_work >= (long long)INT32_MAX || n_spmm_work >= (long long)INT32_MAX ||
       n_tiles >= INT32_MAX || n_coords >= INT32_MAX || n_units >= INT32_MAX ||
-      n >= INT32_MAX)
+      n_blk_work >= (long long)INT32_MAX || n_brows >= (long long)INT32_MAX || n >= INT32_MAX)
     return LOOPS_ERROR_OVERFLOW;
   const size_t tile_ws = n_tiles > 0 ? a256(fixup_ws_bytes(n_tiles)) : 0;
   const size_t coord_ws = a256((size_t)n_coords * sizeof(loops_coord_t));
@@ -604,16 +694,19 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
   hd.n_tiles = n_tiles;
   hd.n_coords = n_coords;
   hd.n_units = n_units;
+  hd.n_blk_work = n_blk_work;
+  hd.n_brows = n_brows;
+  hd.n_blk = n_blk;
   hd.class_begin[0] = 0;
   for (int q = 0; q < kClasses; ++q) hd.class_begin[q + 1] = hd.class_begin[q] + cls_count[q];
   hd.items_off = a256(sizeof(batch_header));
   hd.prefix_off = hd.items_off + a256((size_t)n * sizeof(batch_item_dev));
   hd.order_off = hd.prefix_off + a256((size_t)BP_COUNT * (n + 1) * sizeof(long long));
   hd.cprefix_off = hd.order_off + a256((size_t)n * sizeof(int));
-  info->table_bytes =
-      (int64_t)(hd.cprefix_off + a256((size_t)(n + 1) * sizeof(long long)));
+  hd.blk_off = hd.cprefix_off + a256((size_t)(n + 1) * sizeof(long long));
+  info->table_bytes = (int64_t)(hd.blk_off + a256((size_t)n_blk * sizeof(blk_exec_args)));
   info->workspace_bytes = (int64_t)(kQueueBytes + tile_ws + coord_ws + spmm_ws_bytes + 256);
-  info->work_items = n_spmv_work + n_spmm_work;
+  info->work_items = n_spmv_work + n_spmm_work + n_blk_work;
   info->spmv_tiles = n_tiles;
   info->spmm_units = n_units;
   info->spmm_work_items = n_spmm_work;
@@ -621,6 +714,8 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
   info->max_items_per_thread = max_ipt;
   info->spmm_variant_mask = var_mask;
   info->spmv_class_mask = spmv_mask;
+  info->blocks_work_items = n_blk_work;
+  info->blocks_row_pieces = n_brows;
   if (!out_table) return LOOPS_SUCCESS;
 
   // Second pass: fill the table.
@@ -634,21 +729,24 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
   for (int q = 0; q < BP_COUNT; ++q) P[q] = pre + q * (n + 1);
   int* order = reinterpret_cast<int*>(table + hd.order_off);
   long long* CP = reinterpret_cast<long long*>(table + hd.cprefix_off);
+  blk_exec_args* BA = reinterpret_cast<blk_exec_args*>(table + hd.blk_off);
   long long cls_next[kClasses];
   for (int q = 0; q < kClasses; ++q) cls_next[q] = hd.class_begin[q];
   std::vector<long long> pieces((size_t)n);
   fixup_ws all_tiles = n_tiles > 0 ? carve_fixup_ws(dev_ws, n_tiles) : fixup_ws{};
   loops_coord_t* all_coords = reinterpret_cast<loops_coord_t*>(dev_ws + tile_ws);
   char* spmm_base = dev_ws + tile_ws + coord_ws;
-  long long wv = 0, wm = 0, slot = 0, coord = 0, unit = 0;
+  long long wv = 0, wm = 0, slot = 0, coord = 0, unit = 0, brow = 0;
+  int nblk = 0;
   for (int64_t i = 0; i < n; ++i) {
     const loops_batch_item_t& b = items[i];
     (void)count_item(b, &c);  // validated in the first pass
     order[cls_next[c.cls]++] = (int)i;
-    pieces[i] = c.kind == BK_SPMM ? c.spmm_work : c.spmv_work;
+    pieces[i] = c.kind == BK_SPMM ? c.spmm_work : c.kind == BK_BLOCKS ? c.blk_work : c.spmv_work;
     P[BP_TILE][i] = slot;
     P[BP_COORD][i] = coord;
     P[BP_UNIT][i] = unit;
+    P[BP_BROWS][i] = brow;
     batch_item_dev d{};
     d.offsets = b.A.offsets;
     d.indices = b.A.indices;
@@ -669,6 +767,18 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
     d.ti = c.ti;
     d.first_slot = (int)slot;
     d.first_coord = (int)coord;
+    if (c.kind == BK_BLOCKS) {
+      const blk_plan_state& st = b.plan->st.blk;
+      d.offsets = nullptr;
+      d.indices = nullptr;
+      d.values = nullptr;
+      d.rows = st.rows;
+      d.cols = st.cols;
+      d.nnz = st.nnz;
+      d.vtype = st.vtype;
+      d.blk = nblk;
+      BA[nblk++] = blk_args_of(st, b.y);
+    }
     if (c.kind == BK_MERGE) {
       d.fx.carry_row = all_tiles.carry_row + slot;
       d.fx.seg_end = all_tiles.seg_end + slot;
@@ -688,7 +798,22 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
     slot += c.tiles;
     coord += c.coords;
     unit += c.units;
+    brow += c.brows;
     its[i] = d;
+  }
+  // Plan items: those of several segments first (their pieces then start
+  // at multiples of kBlkPiecePad), each group heaviest first (most rounds),
+  // so that the longest streams start early and short ones fill the end of
+  // the launch.
+  {
+    int* b0 = order + hd.class_begin[kBlkClass];
+    int* b1 = order + hd.class_begin[kBlkClass + 1];
+    std::stable_sort(b0, b1, [&](int u, int v) {
+      const blk_plan_state& a = items[u].plan->st.blk;
+      const blk_plan_state& b = items[v].plan->st.blk;
+      if ((a.S > 1) != (b.S > 1)) return a.S > 1;
+      return a.nrounds > b.nrounds;
+    });
   }
   // Pieces in class order: each class's pieces are contiguous.
   long long cp = 0;
@@ -702,6 +827,7 @@ loops_status_t batch_layout_items(const loops_batch_item_t* items, int64_t n,
   P[BP_TILE][n] = slot;
   P[BP_COORD][n] = coord;
   P[BP_UNIT][n] = unit;
+  P[BP_BROWS][n] = brow;
   return LOOPS_SUCCESS;
 }
 }  // namespace
@@ -734,7 +860,7 @@ const spmm_kernel_table g_spmm_kernels;
 // computed once per device under a lock (the server thread and its clients
 // may ask at the same time).
 struct batch_grids {
-  int spmv4 = 0, spmv8 = 0, spmv16 = 0;
+  int spmv4 = 0, spmv8 = 0, spmv16 = 0, blocks = 0;
   int spmm[kSpmmVariants] = {};
 };
 constexpr int kMaxDevices = 64;
@@ -750,7 +876,7 @@ hipError_t resident_grids(batch_grids* out) {
     *out = g_grids[dev];
     return hipSuccess;
   }
-  int cus = 0, a4 = 0, a8 = 0, a16 = 0;
+  int cus = 0, a4 = 0, a8 = 0, a16 = 0, ab = 0;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a4, batch_spmv_kernel<4>, kBatchTpb, 0);
@@ -758,6 +884,8 @@ hipError_t resident_grids(batch_grids* out) {
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a8, batch_spmv_kernel<8>, kBatchTpb, 0);
   if (e == hipSuccess)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a16, batch_spmv_kernel<16>, kBatchTpb, 0);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&ab, batch_blocks_kernel, kBlkTpb, 0);
   if (e != hipSuccess) return e;
   batch_grids g;
   for (int v = 0; v < kSpmmVariants; ++v) {
@@ -770,6 +898,7 @@ hipError_t resident_grids(batch_grids* out) {
   g.spmv4 = (a4 > 0 ? a4 : 1) * cus;
   g.spmv8 = (a8 > 0 ? a8 : 1) * cus;
   g.spmv16 = (a16 > 0 ? a16 : 1) * cus;
+  g.blocks = (ab > 0 ? ab : 1) * cus;
   if (dev >= 0 && dev < kMaxDevices) g_grids[dev] = g;
   *out = g;
   return hipSuccess;
@@ -787,11 +916,19 @@ hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
   batch_grids rg;
   hipError_t e = resident_grids(&rg);
   if (e != hipSuccess) return e;
-  const long long spmv_work = info->work_items - info->spmm_work_items;
+  const long long spmv_work =
+      info->work_items - info->spmm_work_items - info->blocks_work_items;
   if (info->work_items > 0) {
     const long long pg = info->coordinates > 0 ? (info->coordinates + 255) / 256 : 1;
     static_assert(kQueueSets * kQueues <= 256, "prologue's first block zeroes the counters");
     batch_prologue_kernel<<<(unsigned)pg, 256, 0, s>>>(t);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (info->blocks_work_items > 0) {
+    long long g = grid > 0 ? grid : rg.blocks;
+    if (g > info->blocks_work_items) g = info->blocks_work_items;
+    batch_blocks_kernel<<<(unsigned)g, kBlkTpb, 0, s>>>(t);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -820,6 +957,11 @@ hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
   }
   if (info->spmm_units > 0) {
     batch_spmm_fixup_kernel<<<(unsigned)((info->spmm_units + 255) / 256), 256, 0, s>>>(t);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (info->blocks_row_pieces > 0) {
+    batch_blocks_combine_kernel<<<(unsigned)info->blocks_row_pieces, 256, 0, s>>>(t);
     return hipGetLastError();
   }
   return hipSuccess;

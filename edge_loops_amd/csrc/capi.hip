@@ -13,11 +13,6 @@
 
 using namespace loops_mi355x;
 
-struct loops_spmv_plan {
-  plan_state st;
-  int64_t rows = 0, cols = 0;
-};
-
 struct loops_handle {
   int device = 0;
   int cu_count = 0;
@@ -467,6 +462,9 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
       p->layout > LOOPS_PLAN_LAYOUT_BLOCKS || p->threads_per_block < 0 ||
       p->items_per_thread < 0)
     return LOOPS_ERROR_INVALID_VALUE;
+  const bool panel_param = p->panels != 0 || p->threads_per_block != 0 || p->items_per_thread != 0;
+  if (p->for_batches && (p->layout == LOOPS_PLAN_LAYOUT_PANELS || panel_param))
+    return LOOPS_ERROR_INVALID_VALUE;
   int layout = p->layout;
   // AUTO: row blocks from 2^12 columns and 2^19 nonzeros up, unless a
   // PANELS parameter is set.  Blocks against panels: 200k x 200k, 2M
@@ -474,8 +472,7 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
   // -> 14.5; 30k x 30k, 600k 13.0 -> 11.3; narrow uniform matrices (40 or
   // 1,000 columns) and products under 2^19 nonzeros tie or favour panels.
   if (layout == LOOPS_PLAN_LAYOUT_AUTO)
-    layout = a.cols >= (1 << 12) && a.nnz >= (1 << 19) && p->panels == 0 &&
-                     p->threads_per_block == 0 && p->items_per_thread == 0
+    layout = p->for_batches || (a.cols >= (1 << 12) && a.nnz >= (1 << 19) && !panel_param)
                  ? LOOPS_PLAN_LAYOUT_BLOCKS
                  : LOOPS_PLAN_LAYOUT_PANELS;
   if (layout == LOOPS_PLAN_LAYOUT_BLOCKS) {
@@ -485,7 +482,7 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
     pl->st.rows = (int)A->rows;
     pl->rows = A->rows;
     pl->cols = A->cols;
-    const hipError_t e = blk_plan_build(a, &pl->st.blk, stream);
+    const hipError_t e = blk_plan_build(a, &pl->st.blk, p->for_batches != 0, stream);
     if (e != hipSuccess) {
       blk_plan_release(&pl->st.blk);
       delete pl;

@@ -113,6 +113,7 @@ constexpr int kBlkSlotBits = 12;
 constexpr int kBlkRound = 1 << kBlkSlotBits;    // entries per round at most
 constexpr int kBlkColBits = 32 - kBlkSlotBits;  // column bits within a segment
 constexpr int kBlkChunkCols = 64;               // columns per build bucket
+constexpr int kBlkBatchRounds = 8;              // rounds per stream of plans for batches
 struct blk_plan_state {
   int rows = 0, cols = 0, nnz = 0, S = 4, W = 1, nb = 0, nrounds = 0;
   loops_value_type_t vtype = LOOPS_VALUE_F32;
@@ -146,11 +147,18 @@ struct blk_plan_state {
   // rows of the block (virtual row - blk_start[b]).
   int* rstart = nullptr;
   int* rlist = nullptr;
+  // One segment, no row cut and every row a stream row: the streams write y
+  // themselves (partial is null and no combine runs).
+  bool direct = false;
   void* mem = nullptr;
   size_t device_bytes = 0;
 };
 int blk_segments(long long cols);
-hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s);
+// for_batches: one segment when the columns fit one, and the fewest row
+// blocks the LDS allows (batch items), instead of enough blocks to fill the
+// GPU alone.
+hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batches,
+                          hipStream_t s);
 hipError_t blk_plan_execute(const blk_plan_state& st, const float* x, float* y,
                             hipEvent_t ev1, hipStream_t s);
 void blk_plan_release(blk_plan_state* st);
@@ -230,3 +238,10 @@ hipError_t launch_mp_spmm(const csr_args& A, int k, const float* B, long long ld
                           hipEvent_t ev0, hipEvent_t ev1, hipStream_t s);
 
 }  // namespace loops_mi355x
+
+// The opaque plan handle of the C ABI (loops_spmv_plan_t); batch items may
+// carry one (loops_batch_item_t.plan).
+struct loops_spmv_plan {
+  loops_mi355x::plan_state st;
+  int64_t rows = 0, cols = 0;
+};

@@ -361,6 +361,7 @@ hipError_t launch_tm_spmm(const csr_args& A, int k, const float* B, long long ld
     case LOOPS_VALUE_F32: return tm_spmm_t<float>(A, k, B, ldb, C, ldc, tpb, ev0, ev1, s);
     case LOOPS_VALUE_BF16: return tm_spmm_t<bf16_t>(A, k, B, ldb, C, ldc, tpb, ev0, ev1, s);
     case LOOPS_VALUE_F16: return tm_spmm_t<f16_t>(A, k, B, ldb, C, ldc, tpb, ev0, ev1, s);
+    default: break;
   }
   return hipErrorInvalidValue;
 }
@@ -378,6 +379,7 @@ hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ld
       case LOOPS_VALUE_F32: e = gm_spmm_v<float>(A, k, B, ldb, C, ldc, sh, tpb, s); break;
       case LOOPS_VALUE_BF16: e = gm_spmm_v<bf16_t>(A, k, B, ldb, C, ldc, sh, tpb, s); break;
       case LOOPS_VALUE_F16: e = gm_spmm_v<f16_t>(A, k, B, ldb, C, ldc, sh, tpb, s); break;
+      default: e = hipErrorInvalidValue; break;
     }
     if (e != hipSuccess) return e;
   }

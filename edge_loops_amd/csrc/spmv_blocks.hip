@@ -61,16 +61,11 @@
 #include <type_traits>
 #include <vector>
 
+#include "blk_stream.hpp"
 #include "device_common.hpp"
 #include "loops_internal.hpp"
 
 namespace loops_mi355x {
-
-constexpr int kBlkTpb = 512;
-constexpr int kBlkIpt = kBlkRound / kBlkTpb;  // entries per lane per phase (8)
-constexpr int kBlkSlab = 64 * kBlkIpt;        // entries of one wavefront per round
-static_assert(kBlkIpt == 8 && kBlkIpt * kBlkTpb == kBlkRound,
-              "a round is one pass of the workgroup, 8 slots (16 bytes of rows) per lane");
 
 // ---------------------------------------------------------------------------
 // Build kernels
@@ -451,301 +446,22 @@ __global__ __launch_bounds__(256) void blk_lane_order_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Execute kernels
+// Execute kernels (the stream and combine code is in blk_stream.hpp)
 // ---------------------------------------------------------------------------
 
-typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-
-// What a lane reads of one round: its phase-1 entries (wavefront w: entries
-// 512 w .. 512 w + 511, lane l: 64 j + l, stored lane-major so that the
-// lane's 8 are consecutive) and the rows of its phase-2 slots 8 t .. 8 t + 7;
-// the wave also reads the rows of the slots just before and after its 512.
+// Workgroup b * S + s runs segment s of block blk_order[b].
 template <typename V>
-struct blk_round_regs {
-  int n;
-  uint32_t pw[kBlkIpt];
-  uint32_t vw[kBlkIpt];  // raw value words (blk_word_value)
-  u16x8 rows;
-  int edge_prev, edge_next;  // wave-uniform: the rows of the slots just before / after the wave's
-};
-
-// Values travel as raw 32-bit words and are converted where they are used,
-// so no load result is waited for inside the guarded load block.  A lane's
-// 8 16-bit values are 4 words; value j is half j & 1 of word j / 2.
-template <typename V>
-__device__ __forceinline__ float blk_word_value(uint32_t w, int j) {
-  if constexpr (std::is_same<V, float>::value) {
-    return __uint_as_float(w);
-  } else {
-    const uint32_t h = (j & 1) ? (w >> 16) : (w & 0xffffu);
-    if constexpr (std::is_same<V, bf16_t>::value) return __uint_as_float(h << 16);
-    else return static_cast<float>(__builtin_bit_cast(f16_t, (uint16_t)h));
-  }
+__global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(blk_exec_args a,
+                                                              const float* __restrict__ x) {
+  __shared__ blk_smem sm;
+  blk_stream<V>(a, x, (int)blockIdx.x, sm);
 }
 
-template <typename V>
-__device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict__ packed,
-                                               const V* __restrict__ vals,
-                                               const uint16_t* __restrict__ srow,
-                                               const int* __restrict__ round_off,
-                                               const int* __restrict__ round_base, int lane,
-                                               int wave, int k0, blk_round_regs<V>& g) {
-  g.n = round_off[r + 1] - round_off[r];
-#pragma unroll
-  for (int j = 0; j < kBlkIpt; ++j) {
-    g.pw[j] = 0;
-    g.vw[j] = 0;
-  }
-  // The wave's slab exists when its first entry does; positions past the
-  // round's end inside it are zero padding.
-  if (wave * kBlkSlab < g.n) {
-    const long long s0 = ((long long)round_base[r] + wave) * kBlkSlab;
-    const long long p = s0 + lane * 4, p2 = p + kBlkSlab / 2;
-    const i32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p));
-    const i32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p2));
-    g.pw[0] = c0.x; g.pw[1] = c0.y; g.pw[2] = c0.z; g.pw[3] = c0.w;
-    g.pw[4] = c1.x; g.pw[5] = c1.y; g.pw[6] = c1.z; g.pw[7] = c1.w;
-    if constexpr (std::is_same<V, float>::value) {
-      const i32x4 v0 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(vals + p));
-      const i32x4 v1 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(vals + p2));
-      g.vw[0] = v0.x; g.vw[1] = v0.y; g.vw[2] = v0.z; g.vw[3] = v0.w;
-      g.vw[4] = v1.x; g.vw[5] = v1.y; g.vw[6] = v1.z; g.vw[7] = v1.w;
-    } else {
-      const i32x4 v = __builtin_nontemporal_load(
-          reinterpret_cast<const i32x4*>(vals + s0 + (long long)lane * kBlkIpt));
-      g.vw[0] = g.vw[1] = v.x; g.vw[2] = g.vw[3] = v.y;
-      g.vw[4] = g.vw[5] = v.z; g.vw[6] = g.vw[7] = v.w;
-    }
-  }
-  const uint16_t* rs = srow + (long long)r * kBlkRound;
-  g.rows = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-  if (k0 < g.n) g.rows = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(rs + k0));
-  // The neighbours' rows inside the wave come from lane shuffles (phase 2);
-  // only the two slots at the wave's edges are loaded, as aligned words at
-  // wave-uniform addresses.
-  const uint32_t* rw = reinterpret_cast<const uint32_t*>(rs);
-  const int base = __builtin_amdgcn_readfirstlane(wave) * kBlkSlab;
-  g.edge_prev = -1;
-  g.edge_next = -1;
-  if (base > 0 && base < g.n) g.edge_prev = (int)(rw[(base - 2) >> 1] >> 16);
-  if (base + kBlkSlab < g.n) g.edge_next = (int)(rw[(base + kBlkSlab) >> 1] & 0xffffu);
-}
-
-// Segmented inclusive scan over a wavefront with DPP lane moves: shifts by
-// 1, 2, 4 and 8 inside rows of 16 lanes, then each row takes the total of
-// the rows before it (row_bcast 15 into rows 1 and 3, row_bcast 31 into
-// rows 2 and 3).  A lane whose source is outside the move receives (0, 0),
-// the identity of the combination (v, f) + (v', f') = (f' ? v' : v + v',
-// f | f').
-__device__ __forceinline__ void blk_wave_scan(float& v, int& f) {
-#define BLK_SCAN_STEP(CTRL, ROWS)                                                          \
-  {                                                                                        \
-    const float vu =                                                                       \
-        __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, \
-                                                   false));                                \
-    const int fu = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, false);              \
-    if (!f) v = vu + v;                                                                    \
-    f |= fu;                                                                               \
-  }
-  BLK_SCAN_STEP(0x111, 0xf)  // row_shr:1
-  BLK_SCAN_STEP(0x112, 0xf)  // row_shr:2
-  BLK_SCAN_STEP(0x114, 0xf)  // row_shr:4
-  BLK_SCAN_STEP(0x118, 0xf)  // row_shr:8
-  BLK_SCAN_STEP(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
-  BLK_SCAN_STEP(0x143, 0xc)  // row_bcast:31 into rows 2 and 3
-#undef BLK_SCAN_STEP
-}
-
-// Segmented inclusive scan over the workgroup (block_seg_scan's result):
-// wavefront scans, then each lane combines the totals of the wavefronts
-// before its own, read with 16-byte LDS reads.  One barrier; every lane
-// must call it.  The totals are next written in the following round's scan,
-// after that round's phase-1 barrier, which every wave reaches only once it
-// has read them here.
-__device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int* s_f,
-                                               float& pv, int& pf) {
-  constexpr int NW = kBlkTpb / 64;
-  static_assert(NW == 4 || NW == 8, "four or eight wavefront totals, 16-byte reads");
-  blk_wave_scan(v, f);
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 63) {
-    s_v[wave] = v;
-    s_f[wave] = f;
-  }
-  __syncthreads();
-  float wv[NW];
-  int wf[NW];
-#pragma unroll
-  for (int h = 0; h < NW; h += 4) {
-    const f32x4 vv = *reinterpret_cast<const f32x4*>(s_v + h);
-    const i32x4 ff = *reinterpret_cast<const i32x4*>(s_f + h);
-    wv[h] = vv.x; wv[h + 1] = vv.y; wv[h + 2] = vv.z; wv[h + 3] = vv.w;
-    wf[h] = ff.x; wf[h + 1] = ff.y; wf[h + 2] = ff.z; wf[h + 3] = ff.w;
-  }
-  pv = 0.f;
-  pf = 0;
-#pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    if (w < wave) {
-      pv = wf[w] ? wv[w] : pv + wv[w];
-      pf |= wf[w];
-    }
-  }
-  if (!f) v = pv + v;
-  f |= pf;
-}
-
-template <typename V>
-__global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(
-    const uint32_t* __restrict__ packed, const V* __restrict__ vals,
-    const uint16_t* __restrict__ srow, const int* __restrict__ stream_round,
-    const int* __restrict__ round_off, const int* __restrict__ round_base,
-    const int* __restrict__ blk_start, const int* __restrict__ blk_order,
-    const int* __restrict__ rstart, const int* __restrict__ rlist, int nb,
-    const float* __restrict__ x, int S, int W,
-    float* __restrict__ partial, long long part_stride) {
-  __shared__ float acc[kBlkRows];
-  __shared__ __align__(16) float stage[kBlkRound];
-  __shared__ __align__(16) float s_v[kBlkTpb / 64];
-  __shared__ __align__(16) int s_f[kBlkTpb / 64];
-  const int s = (int)(blockIdx.x % (unsigned)S);
-  const int b = blk_order[blockIdx.x / (unsigned)S];
-  const int q = b * S + s;  // stream
-  // stream rows: the compressed list positions rs .. rs + nrows - 1, or the
-  // block's rows r0 .. r0 + nrows - 1 (<= kBlkRows either way)
-  int r0 = 0, rs = 0, nrows;
-  if (rlist) {
-    const int* rp = rstart + (long long)s * (nb + 1) + b;
-    rs = rp[0];
-    nrows = rp[1] - rs;
-  } else {
-    r0 = blk_start[b];
-    nrows = blk_start[b + 1] - r0;
-  }
-  const float* __restrict__ xs = x + (long long)s * W;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k0 = threadIdx.x * kBlkIpt;
-  const int rb = stream_round[q], re = stream_round[q + 1];
-  blk_round_regs<V> cur;
-  if (rb < re)
-    blk_load_round<V>(rb, packed, vals, srow, round_off, round_base, lane, wave, k0, cur);
-  // the accumulator is cleared while the first round's loads are in flight
-  for (int i = threadIdx.x; i < nrows; i += kBlkTpb) acc[i] = 0.f;
-  for (int r = rb; r < re; ++r) {
-    const int n = cur.n;
-    // Phase 1: products in column order to their slots; the next round's
-    // loads go out behind this round's x reads.
-    float xv[kBlkIpt];
-#pragma unroll
-    for (int j = 0; j < kBlkIpt; ++j) {
-      const int i = wave * 64 * kBlkIpt + j * 64 + lane;
-      xv[j] = i < n ? xs[cur.pw[j] >> kBlkSlotBits] : 0.f;
-    }
-    blk_round_regs<V> nxt;
-    if (r + 1 < re)
-      blk_load_round<V>(r + 1, packed, vals, srow, round_off, round_base, lane, wave, k0, nxt);
-#pragma unroll
-    for (int j = 0; j < kBlkIpt; ++j) {
-      const int i = wave * 64 * kBlkIpt + j * 64 + lane;
-      if (i < n)
-        stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], j) * xv[j];
-    }
-    __syncthreads();
-    // Phase 2: runs of equal rows over slots k0 .. k0 + m - 1.  prev_row /
-    // next_row: the rows of slots k0 - 1 and k0 + kBlkIpt (-1 outside the
-    // round), from the neighbouring lanes or the wave's edge slots.
-    const int m = n - k0 < 0 ? 0 : (n - k0 > kBlkIpt ? kBlkIpt : n - k0);
-    const int up_row = __shfl_up((int)cur.rows[kBlkIpt - 1], 1, 64);
-    const int down_row = __shfl_down((int)cur.rows[0], 1, 64);
-    const int prev_row = k0 == 0 ? -1 : (lane == 0 ? cur.edge_prev : up_row);
-    const int next_row = k0 + kBlkIpt >= n ? -1 : (lane == 63 ? cur.edge_next : down_row);
-    float v = 0.f, first_sum = 0.f;
-    int f = 1, first_row = -1, row = cur.rows[0];
-    bool internal = false;
-    // the lane's 8 slots with two 16-byte LDS reads (slots past the round's
-    // end hold stale products and are not used)
-    const f32x4 sa = *reinterpret_cast<const f32x4*>(stage + k0);
-    const f32x4 sb = *reinterpret_cast<const f32x4*>(stage + k0 + 4);
-    const float sv[kBlkIpt] = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y, sb.z, sb.w};
-    if (m > 0) {
-      v = sv[0];
-#pragma unroll
-      for (int i = 1; i < kBlkIpt; ++i) {
-        if (i < m) {
-          const float p = sv[i];
-          const int ri = cur.rows[i];
-          if (ri != row) {
-            if (!internal) {
-              first_sum = v;
-              first_row = row;
-              internal = true;
-            } else {
-              acc[row] += v;  // a run that starts and ends in this lane
-            }
-            row = ri;
-            v = p;
-          } else {
-            v += p;
-          }
-        }
-      }
-      f = internal || prev_row != (int)cur.rows[0];
-    }
-    const float v_own = v;
-    float pv;
-    int pf;
-    blk_block_scan(v, f, s_v, s_f, pv, pf);
-    // exclusive: the running sum of the previous lane's trailing run
-    const float up = __shfl_up(v, 1, 64);
-    const float ex = lane == 0 ? pv : up;
-    if (m > 0) {
-      const bool cont = k0 > 0 && prev_row == (int)cur.rows[0];
-      const float cin = cont ? ex : 0.f;
-      if (internal) acc[first_row] += cin + first_sum;
-      if (k0 + m >= n || next_row != row) acc[row] += (internal ? 0.f : cin) + v_own;
-    }
-    if (r + 1 < re) cur = nxt;
-  }
-  __syncthreads();
-  float* out = partial + (long long)s * part_stride;
-  if (rlist) {
-    for (int i = threadIdx.x; i < nrows; i += kBlkTpb)
-      __builtin_nontemporal_store(acc[i], out + rlist[rs + i]);
-  } else {
-    for (int i = threadIdx.x; i < nrows; i += kBlkTpb)
-      __builtin_nontemporal_store(acc[i], out + r0 + i);
-  }
-}
-
-// Sum of the S partials of virtual row v in segment order, loads four at a
-// time (S is 4 or a multiple of 8).
-__device__ __forceinline__ float blk_partials(const float* __restrict__ partial,
-                                              long long part_stride, int S, long long v) {
-  float sum = 0.f;
-  for (int s0 = 0; s0 < S; s0 += 4) {
-    float g[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) g[u] = partial[(long long)(s0 + u) * part_stride + v];
-    sum = s0 == 0 ? ((g[0] + g[1]) + g[2]) + g[3] : (((sum + g[0]) + g[1]) + g[2]) + g[3];
-  }
-  return sum;
-}
-
-// y[r] = ((partial_0[r] + partial_1[r]) + ...) + partial_{S-1}[r]; a row cut
-// into pieces adds its pieces' sums in piece order.
 __global__ __launch_bounds__(256) void spmv_blocks_combine_kernel(
     const float* __restrict__ partial, long long part_stride, int S, int rows,
     const int* __restrict__ vrow, float* __restrict__ y) {
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
-    if (!vrow) {
-      y[r] = blk_partials(partial, part_stride, S, r);
-      continue;
-    }
-    const int v0 = vrow[r], v1 = vrow[r + 1];
-    float sum = blk_partials(partial, part_stride, S, v0);
-    for (int v = v0 + 1; v < v1; ++v) sum += blk_partials(partial, part_stride, S, v);
-    y[r] = sum;
-  }
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x)
+    y[r] = blk_row_sum(partial, part_stride, S, vrow, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -771,11 +487,9 @@ hipError_t blk_scatter(const csr_args& A, const blk_geom& g, const int* cnt, int
 }
 
 template <typename V>
-hipError_t blk_execute(const blk_plan_state& st, const float* x, hipStream_t s) {
+hipError_t blk_execute(const blk_plan_state& st, const float* x, float* y, hipStream_t s) {
   spmv_blocks_kernel<V><<<(unsigned)((long long)st.nb * st.S), kBlkTpb, 0, s>>>(
-      st.packed, static_cast<const V*>(st.values), st.srow, st.stream_round, st.round_off,
-      st.round_base, st.blk_start, st.blk_order, st.rstart, st.rlist, st.nb, x, st.S, st.W,
-      st.partial, st.part_stride);
+      blk_args_of(st, y), x);
   return hipGetLastError();
 }
 }  // namespace
@@ -788,13 +502,14 @@ int blk_segments(long long cols) {
   return (int)(8 * ((cols + 8 * per - 1) / (8 * per)));
 }
 
-hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) {
+hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batches,
+                          hipStream_t s) {
   blk_plan_state& P = *st;
   P.rows = A.rows;
   P.cols = A.cols;
   P.nnz = A.nnz;
   P.vtype = A.vtype;
-  P.S = blk_segments(A.cols);
+  P.S = for_batches && A.cols <= (1LL << kBlkColBits) ? 1 : blk_segments(A.cols);
   P.W = (int)((A.cols + P.S - 1) / P.S);
   if (P.W < 1) P.W = 1;
   if (A.rows == 0) return hipSuccess;  // nothing to store; execution only records ev1
@@ -810,7 +525,17 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   // than 1.2 x is redone for two waves.  The target grows until the cut yields no more
   // blocks than wanted.  A row longer than an eighth of the one-wave share
   // (and than a round) is cut into pieces, entry i of m to piece i % m, so
-  // that every piece spans the columns as the row does.
+  // that every piece spans the columns as the row does.  Plans for batch
+  // items aim at the fewest blocks the row capacity allows instead (the
+  // batch fills the GPU), with shares and the piece length to match, and
+  // are never compressed when they have one segment.  A batch's persistent
+  // workgroups take streams one after another, so a stream is kept to about
+  // kBlkBatchRounds rounds: on 512 power-law matrices of 1e3-2e5 rows
+  // (Zipf(1.8) row lengths, 253M nonzeros, plans for the 367 of 2^17 or more
+  // nonzeros) a batch pass took 1.42 ms with blocks of the full row
+  // capacity (streams of up to 54 rounds: the last ones ran alone), 1.21
+  // with 32 rounds, 1.01 with 16, 0.94 with 8, 0.96 with 4; merge-path
+  // tiles alone 1.28 ms.
   std::vector<int> off_h((size_t)A.rows + 1, 0), starts(1, 0);
   hipError_t e = hipMemcpyAsync(off_h.data(), A.offsets, ((size_t)A.rows + 1) * sizeof(int),
                                 hipMemcpyDeviceToHost, s);
@@ -820,7 +545,13 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   const long long wgs = 2LL * cus * 31 / 32;
-  const long long want1 = wgs / P.S > 0 ? wgs / P.S : 1;  // blocks of one wave
+  // batch items: blocks of about kBlkBatchRounds rounds per stream, or
+  // fewer rows when the LDS row capacity binds
+  const long long per_block = (long long)kBlkBatchRounds * kBlkRound * P.S;
+  const long long fewest = std::max(((long long)A.rows + kBlkRows - 1) / kBlkRows,
+                                    ((long long)A.nnz + per_block - 1) / per_block);
+  const long long want1 = for_batches ? fewest  // the fewest blocks
+                                      : (wgs / P.S > 0 ? wgs / P.S : 1);  // blocks of one wave
   const long long share1 = A.nnz / want1 > 0 ? A.nnz / want1 : 1;
   const long long cap = share1 / 8 > kBlkRound ? share1 / 8 : kBlkRound;
   std::vector<int> vrow_h;
@@ -844,7 +575,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
     }
     vrow_h[A.rows] = v;
   }
-  const bool masks = P.S <= 64 && (long long)P.S * V <= (1LL << 28);
+  const bool masks = P.S > 1 && P.S <= 64 && (long long)P.S * V <= (1LL << 28);
 
   // Early scratch, kept to the end: first virtual row of each row, segment
   // masks of the virtual rows (at most 64 segments).
@@ -943,14 +674,19 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
       for (long long c : rows_in_seg) need = std::max(need, (c + kBlkRows - 1) / kBlkRows);
     }
     const long long slots = 2LL * cus / P.S;  // blocks of one full wave
-    long long want = std::max(want1, need);
-    if (want > slots) want = std::max(2 * want1, need);
-    cut(want);
-    long long most = 0;
-    for (long long n : bnnz) most = n > most ? n : most;
-    if (want <= slots && (double)most * bnnz.size() > 1.2 * (double)A.nnz)
-      cut(std::max(2 * want1, need));
+    if (for_batches) {
+      cut(std::max(need, fewest));
+    } else {
+      long long want = std::max(want1, need);
+      if (want > slots) want = std::max(2 * want1, need);
+      cut(want);
+      long long most = 0;
+      for (long long n : bnnz) most = n > most ? n : most;
+      if (want <= slots && (double)most * bnnz.size() > 1.2 * (double)A.nnz)
+        cut(std::max(2 * want1, need));
+    }
   }
+  P.direct = P.S == 1 && P.nsplit == 0 && !comp;
   P.nb = (int)starts.size() - 1;
   // Launch order: heaviest blocks first.
   std::vector<int> order((size_t)P.nb);
@@ -1075,7 +811,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   const size_t b_vals = al256(n_lane * vbytes);
   const size_t b_srow = al256(((size_t)nrounds * kBlkRound + 8) * sizeof(uint16_t));
   const size_t b_roff = al256((size_t)(nrounds + 1) * sizeof(int));
-  const size_t b_part = al256((size_t)P.S * P.part_stride * sizeof(float));
+  const size_t b_part = P.direct ? 0 : al256((size_t)P.S * P.part_stride * sizeof(float));
   size_t total = 0;
   auto take = [&](size_t bytes) {
     const size_t at = total;
@@ -1096,7 +832,7 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
   P.stream_round = reinterpret_cast<int*>(m + o_sround);
   P.round_off = reinterpret_cast<int*>(m + o_roff);
   P.round_base = reinterpret_cast<int*>(m + o_rbase);
-  P.partial = reinterpret_cast<float*>(m + o_part);
+  P.partial = P.direct ? nullptr : reinterpret_cast<float*>(m + o_part);
   P.blk_start = reinterpret_cast<int*>(m + o_bstart);
   P.blk_order = reinterpret_cast<int*>(m + o_order);
   if (vrow) P.vrow = reinterpret_cast<int*>(m + o_vrow);
@@ -1168,6 +904,9 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
       case LOOPS_VALUE_F16:
         e = blk_scatter<f16_t>(A, g, cnt, start, big + 1, nbig, trow, pk, vv, (unsigned)grid, s);
         break;
+      default:  // storage types of the typed entry points never reach a plan
+        e = hipErrorInvalidValue;
+        break;
     }
   }
   if (e == hipSuccess && nrounds > 0) {
@@ -1192,6 +931,9 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, hipStream_t s) 
             P.round_off, P.round_base, pk, static_cast<const f16_t*>(vv), P.packed,
             static_cast<f16_t*>(P.values));
         break;
+      default:
+        e = hipErrorInvalidValue;
+        break;
     }
     e = hipGetLastError();
   }
@@ -1206,12 +948,14 @@ hipError_t blk_plan_execute(const blk_plan_state& st, const float* x, float* y,
   hipError_t e = hipSuccess;
   if (st.rows == 0) return ev1 ? hipEventRecord(ev1, s) : hipSuccess;
   switch (st.vtype) {
-    case LOOPS_VALUE_F32: e = blk_execute<float>(st, x, s); break;
-    case LOOPS_VALUE_BF16: e = blk_execute<bf16_t>(st, x, s); break;
-    case LOOPS_VALUE_F16: e = blk_execute<f16_t>(st, x, s); break;
+    case LOOPS_VALUE_F32: e = blk_execute<float>(st, x, y, s); break;
+    case LOOPS_VALUE_BF16: e = blk_execute<bf16_t>(st, x, y, s); break;
+    case LOOPS_VALUE_F16: e = blk_execute<f16_t>(st, x, y, s); break;
+    default: e = hipErrorInvalidValue; break;
   }
   if (e != hipSuccess) return e;
   if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
+  if (st.direct) return hipSuccess;  // the streams wrote y
   long long grid = ((long long)st.rows + 255) / 256;
   if (grid > 8192) grid = 8192;
   spmv_blocks_combine_kernel<<<(unsigned)grid, 256, 0, s>>>(st.partial, st.part_stride, st.S,

@@ -334,6 +334,8 @@ hipError_t mp_dispatch_v(const csr_args& A, const float* x, float* y,
       return mp_launch_t<TPB, IPT, bf16_t>(A, x, y, coords, ws, ev0, ev1, s);
     case LOOPS_VALUE_F16:
       return mp_launch_t<TPB, IPT, f16_t>(A, x, y, coords, ws, ev0, ev1, s);
+    default:
+      break;
   }
   return hipErrorInvalidValue;
 }
@@ -443,6 +445,7 @@ hipError_t wo_dispatch_v(const csr_args& A, const float* x, float* y, int grid,
     case LOOPS_VALUE_F32: return wo_launch_t<TPB, float>(A, x, y, grid, ws, ev0, ev1, s);
     case LOOPS_VALUE_BF16: return wo_launch_t<TPB, bf16_t>(A, x, y, grid, ws, ev0, ev1, s);
     case LOOPS_VALUE_F16: return wo_launch_t<TPB, f16_t>(A, x, y, grid, ws, ev0, ev1, s);
+    default: break;
   }
   return hipErrorInvalidValue;
 }
@@ -487,6 +490,7 @@ hipError_t gm_dispatch_v(const csr_args& A, const float* x, float* y,
     case LOOPS_VALUE_F32: return gm_launch_t<TPB, TPT, float>(A, x, y, ev0, ev1, s);
     case LOOPS_VALUE_BF16: return gm_launch_t<TPB, TPT, bf16_t>(A, x, y, ev0, ev1, s);
     case LOOPS_VALUE_F16: return gm_launch_t<TPB, TPT, f16_t>(A, x, y, ev0, ev1, s);
+    default: break;
   }
   return hipErrorInvalidValue;
 }
@@ -557,6 +561,7 @@ hipError_t launch_tm_spmv(const csr_args& A, const float* x, float* y, int tpb,
     case LOOPS_VALUE_F32: return tm_launch_t<float>(A, x, y, tpb, original, ev0, ev1, s);
     case LOOPS_VALUE_BF16: return tm_launch_t<bf16_t>(A, x, y, tpb, original, ev0, ev1, s);
     case LOOPS_VALUE_F16: return tm_launch_t<f16_t>(A, x, y, tpb, original, ev0, ev1, s);
+    default: break;
   }
   return hipErrorInvalidValue;
 }

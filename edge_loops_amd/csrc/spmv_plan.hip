@@ -272,6 +272,8 @@ hipError_t panels_launch(const panel_set_t& ps, loops_value_type_t vt,
     case LOOPS_VALUE_F16:
       spmv_panels_kernel<TPB, IPT, f16_t><<<grid, TPB, 0, s>>>(ps, x, rows, P);
       break;
+    default:  // storage types of the typed entry points never reach a plan
+      return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
@@ -371,6 +373,8 @@ hipError_t plan_build(const csr_args& A, int P, int tpb, int ipt, plan_state* st
               A.offsets, A.indices, static_cast<const f16_t*>(A.values), A.rows,
               A.nnz, P, bound, cnt, poff, pidx, pval);
           break;
+        default:
+          break;  // check_csr admits fp32 / bf16 / fp16 only
       }
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }

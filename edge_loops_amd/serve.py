@@ -31,7 +31,7 @@ def matrix_tunings(csr, table=None, max_row=None):
     if max_row is None:
         import torch
         max_row = int(torch.diff(csr.offsets).max().item()) if csr.rows else 0
-    sched, ipt = table.pick(csr.rows, csr.nnzs, max_row, 1)
+    sched, ipt = table.pick(csr.rows, csr.nnzs, max_row, 1, allow_blocks=False)  # no plans here
     m = _capi.ServerMatrix()
     m.A = csr.view()
     m.spmv_schedule = _capi.SCHEDULES[sched]

@@ -7,7 +7,10 @@ group-mapped).  Here the candidates are the batch kernel's piece kinds:
 
   k == 1: merge_path_flat with 4, 8 or 16 steps per lane; thread_mapped;
           group_mapped with G lanes per row (G from the mean row length, and
-          4x that);
+          4x that); "blocks": a row-block plan cut for batches
+          (spmv_plan(..., layout="blocks", for_batches=True)), measured for
+          buckets whose matrices hold at least BLOCKS_MIN_NNZ nonzeros on
+          average;
   k >  1: merge_path_flat SpMM tiles of 4, 8 or 16 merge steps per range.
 
 For each bucket the tuning matrices that fall in it are run as one batch per
@@ -17,6 +20,10 @@ table is plain JSON: bucket key -> {"schedule", "tile", "us": {candidate:
 microseconds}}.  Lookup falls back to the nearest tuned bucket, then to
 merge_path_flat with 8 steps per lane (SpMM: 8 steps per range); a bucket's
 measured winner is used only when it beats that default by PICK_MARGIN.
+A "blocks" pick asks the caller for a plan per matrix (built once; a copy
+of A in the row-block layout); callers that take no plans ask for picks
+without it (allow_blocks=False) and get the best of the other candidates,
+as do matrices of fewer than BLOCKS_MIN_NNZ nonzeros.
 """
 from __future__ import annotations
 
@@ -42,6 +49,9 @@ PICK_MARGIN = 0.05
 # 1.28 ms pass.
 MIN_CLASS_SHARE = 0.1
 TABLE_PATH = os.path.join(os.path.dirname(__file__), "tuning", "mi355x_batch.json")
+# Below this many nonzeros per matrix a plan is a round or two per stream
+# and is not measured.
+BLOCKS_MIN_NNZ = 1 << 14
 
 
 def features(rows, nnz, max_row, k=1):
@@ -75,14 +85,17 @@ def spmv_candidates():
     """(schedule, tile) names; group_mapped tiles are lane-count factors
     over group_lanes() of the matrix ("x1", "x4")."""
     return ([("merge_path_flat", i) for i in SPMV_MERGE]
-            + [("thread_mapped", 0), ("group_mapped", "x1"), ("group_mapped", "x4")])
+            + [("thread_mapped", 0), ("group_mapped", "x1"), ("group_mapped", "x4"),
+               ("blocks", 0)])
 
 
 def kernel_class(sched, ipt, k=1):
-    """The batch kernel an item runs in: ("spmv", 4 | 8 | 16) or ("spmm",
-    steps per range)."""
+    """The batch kernel an item runs in: ("spmv", 4 | 8 | 16), ("blocks",)
+    or ("spmm", steps per range)."""
     if k > 1:
         return ("spmm", ipt)
+    if sched == "blocks":
+        return ("blocks",)
     if sched == "merge_path_flat" and ipt in (8, 16):
         return ("spmv", ipt)
     return ("spmv", 4)
@@ -93,6 +106,8 @@ def resolve(sched, tile, rows, nnz):
     if sched == "group_mapped":
         f = int(str(tile).lstrip("x"))
         return min(64, group_lanes(rows, nnz) * f)
+    if sched == "blocks":
+        return 0
     return int(tile)
 
 
@@ -118,17 +133,17 @@ class Table:
         with open(path, "w") as f:
             json.dump(self.data, f, indent=1, sort_keys=True)
 
-    def pick(self, rows, nnz, max_row, k=1):
+    def pick(self, rows, nnz, max_row, k=1, allow_blocks=True):
         """(schedule, items_per_thread) for one product."""
-        sched, tile = self.pick_name(rows, nnz, max_row, k)
+        sched, tile = self.pick_name(rows, nnz, max_row, k, allow_blocks)
         return sched, resolve(sched, tile, rows, nnz)
 
-    def pick_batch(self, products):
+    def pick_batch(self, products, allow_blocks=True):
         """Picks for the products of one batch launch, products = [(rows,
         nnz, max_row, k)]: each product's pick, except where its kernel
         class holds less than MIN_CLASS_SHARE of the batch's work ((rows +
         nnz) * k) and is not the default's class; those take the default."""
-        picks = [self.pick(r, n, mr, k) for r, n, mr, k in products]
+        picks = [self.pick(r, n, mr, k, allow_blocks) for r, n, mr, k in products]
         work = [(r + n) * max(k, 1) for r, n, _, k in products]
         share = {}
         for (sched, ipt), w, (_, _, _, k) in zip(picks, work, products):
@@ -145,7 +160,7 @@ class Table:
             out.append((sched, ipt))
         return out
 
-    def pick_name(self, rows, nnz, max_row, k=1):
+    def pick_name(self, rows, nnz, max_row, k=1, allow_blocks=True):
         key = features(rows, nnz, max_row, k)
         tab = self.data["spmv" if k == 1 else "spmm"]
         if key in tab:
@@ -157,17 +172,22 @@ class Table:
         else:
             return DEFAULT_SPMV if k == 1 else DEFAULT_SPMM
         default = DEFAULT_SPMV if k == 1 else DEFAULT_SPMM
+        us = e.get("us", {})
+        sched, tile = e["schedule"], e["tile"]
+        if sched == "blocks" and (not allow_blocks or nnz < BLOCKS_MIN_NNZ):
+            rest = {c: v for c, v in us.items() if not c.startswith("blocks:")}
+            if not rest:
+                return default
+            sched, tile = min(rest, key=rest.get).split(":")
         # JSON keeps tiles as strings ("8", "x4"); numeric ones become int.
-        t = e["tile"]
-        t = int(t) if str(t).isdigit() else t
+        t = int(tile) if str(tile).isdigit() else tile
         if k > 1 and t not in SPMM_IPT:
             return default  # an entry from another tile design
-        us = e.get("us", {})
-        best = us.get(f"{e['schedule']}:{e['tile']}")
+        best = us.get(f"{sched}:{tile}")
         base = us.get(f"{default[0]}:{default[1]}")
         if best is not None and base is not None and best > (1.0 - PICK_MARGIN) * base:
             return default
-        return e["schedule"], t
+        return sched, t
 
 
 def tuning_set(count=192, seed=77):
@@ -216,6 +236,7 @@ def tune(mats, ks=(2, 4, 8, 16, 32), min_work=30_000_000, log=print):
     """Measure every candidate per bucket on the GPU; returns a Table."""
     import torch
     from . import containers
+    from .algorithms import spmv_plan
     from .batch import BatchItem
     dev = []
     for m in mats:
@@ -234,15 +255,23 @@ def tune(mats, ks=(2, 4, 8, 16, 32), min_work=30_000_000, log=print):
         rep = min(rep, 64)
         times = {}
         for sched, tile in spmv_candidates():
+            if sched == "blocks" and work - sum(m.rows for m, _ in group) < \
+                    BLOCKS_MIN_NNZ * len(group):
+                continue
             items = []
             for r in range(rep):
                 for m, A in group:
                     if id(A) not in xs:
                         xs[id(A)] = torch.rand(A.cols, device="cuda") * 2 - 1
                     y = torch.empty(A.rows, device="cuda")
-                    items.append(BatchItem(A, xs[id(A)], y, sched,
-                                           resolve(sched, tile, m.rows, m.nnz)))
+                    if sched == "blocks":  # a plan per item: items of one plan would race
+                        plan = spmv_plan(A, layout="blocks", for_batches=True)
+                        items.append(BatchItem(A, xs[id(A)], y, "blocks", plan=plan))
+                    else:
+                        items.append(BatchItem(A, xs[id(A)], y, sched,
+                                               resolve(sched, tile, m.rows, m.nnz)))
             times[f"{sched}:{tile}"] = _time_batch(items)
+            del items
         best = min(times, key=times.get)
         sched, tile = best.split(":")
         table.data["spmv"][key] = {"schedule": sched, "tile": tile, "us": times,
@@ -286,11 +315,15 @@ def main(argv=None):
     ap.add_argument("--out", default=TABLE_PATH)
     ap.add_argument("--count", type=int, default=192)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="tune the SpMV buckets only; keep the SpMM entries of --out")
     a = ap.parse_args(argv)
     mats = tuning_set(24 if a.quick else a.count)
     print(f"tuning set: {len(mats)} matrices, {sum(m.nnz for m in mats)} nnz", flush=True)
-    t = tune(mats, ks=(4, 32) if a.quick else (2, 4, 8, 16, 32),
-             log=lambda s: print(s, flush=True))
+    ks = () if a.spmv_only else (4, 32) if a.quick else (2, 4, 8, 16, 32)
+    t = tune(mats, ks=ks, log=lambda s: print(s, flush=True))
+    if a.spmv_only:
+        t.data["spmm"] = Table.load(a.out).data.get("spmm", {})
     t.data["about"] = {
         "device": "MI355X (gfx950)",
         "features": "k{ceil log2 k}/s{floor log2(rows+nnz) // 2}/d{floor log2 mean row}"

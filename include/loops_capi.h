@@ -170,7 +170,7 @@ loops_status_t loops_spmv_noalloc(loops_handle_t handle, loops_schedule_t schedu
 
 /* y = A * x with A->value_type LOOPS_VALUE_F64 (x, y double) or
  * LOOPS_VALUE_I32 (x, y int32): the reference's host entries for
- * type_t = double and int (algorithms/spmv/*.cuh; its benchmark's value
+ * type_t = double and int (algorithms/spmv/<schedule>.cuh; its benchmark's value
  * types, benchmarks/spmv/work_oriented.cu:53).  Same schedules, partitions
  * and contract as loops_spmv (y overwritten, fixed-order sums); default
  * tiles follow the reference per value size: merge_path_flat 64 x 3 for
@@ -251,6 +251,17 @@ typedef struct loops_plan_params_t {
   int32_t items_per_thread;  /* PANELS: 0 = 4                               */
   int32_t panels;            /* PANELS: 1..8; 0 = one per 2 MiB of x, <= 8  */
   int32_t layout;            /* loops_plan_layout_t                         */
+  int32_t for_batches;       /* BLOCKS (AUTO then means BLOCKS): nonzero cuts
+                                the plan for batch items (loops_batch_item_t
+                                .plan) instead of for a launch of its own:
+                                one column segment when cols <= 2^20, and
+                                as few row blocks as the LDS holds, so each
+                                workgroup streams many full rounds; with one
+                                segment and no row cut into pieces the
+                                streams write y themselves (no partial sums,
+                                no combine).  Alone such a plan underfills
+                                the GPU.  Not valid with PANELS or a PANELS
+                                parameter.                                  */
 } loops_plan_params_t;
 
 loops_status_t loops_spmv_plan_create(loops_handle_t handle,
@@ -293,7 +304,14 @@ loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
  *           length), reduced in a fixed order;
  *   k >  1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge-path units of lane
  *           groups spanning the k columns, items_per_thread steps per group
- *           (4, 8 or 16; 0 = the loops_spmm default).
+ *           (4, 8 or 16; 0 = the loops_spmm default);
+ *   k == 1, plan != NULL: the row-block streams of a BLOCKS plan of A
+ *           (loops_spmv_plan_create), bit for bit what
+ *           loops_spmv_plan_execute gives; schedule, items_per_thread and
+ *           A's pointers are ignored, A's rows and cols must be the plan's.
+ *           The launch uses the plan's partial sums: a plan may appear in
+ *           one item of a batch only, must outlive the batch's launches,
+ *           and those launches must not overlap other executions of it.
  * Values fp32, bf16 or fp16; indices / values 16-byte aligned (8 for 16-bit
  * values).
  * Steps: loops_batch_query (sizes) -> loops_batch_prepare (host only: fills
@@ -311,6 +329,7 @@ typedef struct loops_batch_item_t {
   int64_t ldy;
   loops_schedule_t schedule;
   int32_t items_per_thread;
+  loops_spmv_plan_t plan; /* NULL, or a BLOCKS plan of A (k == 1) */
 } loops_batch_item_t;
 
 typedef struct loops_batch_info_t {
@@ -327,6 +346,11 @@ typedef struct loops_batch_info_t {
   int64_t spmv_class_mask;   /* bit c: SpMV pieces of class c exist (0: merge
                                 tiles of 4 steps, thread / group mapped; 1:
                                 8 steps; 2: 16 steps); one kernel per bit   */
+  int64_t blocks_work_items; /* (block, segment) streams of the plan items
+                                (of work_items), those of plans of several
+                                segments padded per item to a multiple of
+                                8                                          */
+  int64_t blocks_row_pieces; /* 256-row pieces of the plan items' combine  */
 } loops_batch_info_t;
 
 loops_status_t loops_batch_query(const loops_batch_item_t* items, int64_t n,

@@ -126,7 +126,7 @@ def test_committed_table_is_complete():
     assert t.data["spmv"] and t.data["spmm"]
     for sec in ("spmv", "spmm"):
         for key, e in t.data[sec].items():
-            assert e["schedule"] in _capi.SCHEDULES
+            assert e["schedule"] in _capi.SCHEDULES or (sec == "spmv" and e["schedule"] == "blocks")
             sched, ipt = t.pick(1000, 5000, 50, 1 if sec == "spmv" else 8)
             assert sched in _capi.SCHEDULES and ipt >= 0
 
@@ -174,3 +174,27 @@ def test_pick_margin_keeps_the_default_for_small_gains():
     t.data["spmm"][kkey] = {"schedule": "merge_path_flat", "tile": "4",
                             "us": {"merge_path_flat:4": 99.0, "merge_path_flat:8": 100.0}}
     assert t.pick(1000, 12000, 400, 16) == tune.DEFAULT_SPMM
+
+
+def test_blocks_picks():
+    """A bucket won by row-block plans: picked when allowed and the matrix
+    holds enough nonzeros; otherwise the best other candidate (subject to
+    the margin); folded into the default like any small kernel class."""
+    t = tune.Table()
+    key = tune.features(100000, 1400000, 4096, 1)
+    us = {"merge_path_flat:4": 120.0, "merge_path_flat:8": 100.0, "merge_path_flat:16": 93.0,
+          "blocks:0": 70.0}
+    t.data["spmv"][key] = {"schedule": "blocks", "tile": "0", "us": us}
+    assert t.pick(100000, 1400000, 4096) == ("blocks", 0)
+    assert t.pick(100000, 1400000, 4096, allow_blocks=False) == ("merge_path_flat", 16)
+    us["merge_path_flat:16"] = 97.0  # the best other one within the margin: the default
+    assert t.pick(100000, 1400000, 4096, allow_blocks=False) == tune.DEFAULT_SPMV
+    # nearest bucket for a matrix too small for a plan
+    assert t.pick(2000, tune.BLOCKS_MIN_NNZ - 1, 100)[0] != "blocks"
+    assert tune.kernel_class("blocks", 0) == ("blocks",)
+    other = (200000, 200000, 1, 1)
+    t.data["spmv"][tune.features(*other)] = {"schedule": "merge_path_flat", "tile": "8"}
+    big = (100000, 1400000, 4096, 1)
+    assert t.pick_batch([big, other])[0] == ("blocks", 0)
+    assert t.pick_batch([big] + [other] * 40)[0] == tune.DEFAULT_SPMV  # 3.5 % of the work
+    assert tune.resolve("blocks", "0", 10, 10) == 0

@@ -99,13 +99,26 @@ def _check(y, off, idx, val, x):
     assert bad == 0, worst
 
 
+def segments(cols, for_batches=False):
+    """Column segments of a blocks plan: 4 while 4 segments of 2^20 columns
+    hold the columns, else 8; plans cut for batches take one up to 2^20."""
+    if for_batches and cols <= 2 ** 20:
+        return 1
+    return 4 if cols <= 4 * 2 ** 20 else 8
+
+
+@pytest.mark.parametrize("for_batches", [False, True], ids=["alone", "for_batches"])
 @pytest.mark.parametrize("name", list(MATS))
-def test_blocks_parity(name):
+def test_blocks_parity(name, for_batches):
     off, idx, val, cols = MATS[name]
     A = loops.csr_t.from_host(off.shape[0] - 1, cols, off, idx, val)
-    plan = loops.spmv_plan(A, layout="blocks")
+    plan = loops.spmv_plan(A, layout="blocks", for_batches=for_batches)
     assert plan.layout == "blocks"
-    assert plan.panels == (4 if cols <= 4 * 2 ** 20 else 8)
+    assert plan.panels == segments(cols, for_batches)
+    if for_batches:  # the fewest blocks the LDS holds, not enough to fill the GPU
+        alone = loops.spmv_plan(A, layout="blocks")
+        assert plan.blocks <= alone.blocks
+        alone.destroy()
     x_np = np.random.default_rng(3).uniform(-1, 1, cols).astype(np.float32)
     x = torch.from_numpy(x_np).cuda()
     ys = []

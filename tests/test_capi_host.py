@@ -102,10 +102,54 @@ def test_plan_parameter_validation_without_a_device():
     h = ctypes.c_void_p()
     L = _capi.lib()
     for bad in (_capi.PlanParams(0, 0, 0, 3), _capi.PlanParams(0, 0, 0, -1),
-                _capi.PlanParams(0, 0, 9, 1), _capi.PlanParams(-1, 0, 0, 1)):
+                _capi.PlanParams(0, 0, 9, 1), _capi.PlanParams(-1, 0, 0, 1),
+                # a blocks cut for batches with PANELS, or with a PANELS parameter
+                _capi.PlanParams(0, 0, 0, 1, 1), _capi.PlanParams(0, 0, 2, 0, 1),
+                _capi.PlanParams(256, 0, 0, 2, 1)):
         st = L.loops_spmv_plan_create(None, ctypes.byref(v), ctypes.byref(bad), None,
                                       ctypes.byref(h))
         assert st == _capi.ERROR_INVALID_VALUE
     assert L.loops_spmv_plan_layout(None, ctypes.byref(ctypes.c_int32())) == \
         _capi.ERROR_INVALID_VALUE
     assert _capi.PLAN_LAYOUTS == {"auto": 0, "panels": 1, "blocks": 2}
+
+
+STRUCTS = {  # ctypes mirror -> C typedef of include/loops_capi.h
+    "CsrView": "loops_csr_view_t", "Coord": "loops_coord_t", "Params": "loops_params_t",
+    "PlanParams": "loops_plan_params_t", "BatchItem": "loops_batch_item_t",
+    "BatchInfo": "loops_batch_info_t", "ServerConfig": "loops_server_config_t",
+    "ServerMatrix": "loops_server_matrix_t", "ServerStats": "loops_server_stats_t",
+    "ServerRequest": "loops_server_request_t",
+}
+
+
+def test_ctypes_mirrors_match_the_header(tmp_path):
+    """Size and every field offset of each ctypes structure equal the C
+    header's, as gcc lays them out (a field added on one side only shifts
+    what the library reads)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("gcc or the HIP headers are absent")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", "#include <loops_capi.h>",
+             "int main(void) {"]
+    expect = []
+    for py, c in STRUCTS.items():
+        cls = getattr(_capi, py)
+        lines.append(f'  printf("%zu\\n", sizeof({c}));')
+        expect.append((f"sizeof {c}", ctypes.sizeof(cls)))
+        for f in cls._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({c}, {f[0]}));')
+            expect.append((f"{c}.{f[0]}", getattr(cls, f[0]).offset))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert len(got) == len(expect)
+    bad = [(name, want, g) for (name, want), g in zip(expect, got) if want != g]
+    assert not bad, bad

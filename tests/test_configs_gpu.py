@@ -2,9 +2,11 @@
 
 Batched: the 512-matrix set (rows log-uniform in [1e3, 2e5], nnz/row
 Zipf(1.8) capped at 4096, fp32), one persistent launch with the autotuner's
-picks, captured in a graph and replayed; every row of every matrix within
-1e-5 of sum_j |a_ij x_j| of the FP64 oracle, each replay bit-identical to an
-eager launch, and items bit-identical to the same product launched alone.
+picks (merge-path tiles and row-block plans cut for batches), captured in a
+graph and replayed; every row of every matrix within 1e-5 of sum_j |a_ij
+x_j| of the FP64 oracle, each replay bit-identical to an eager launch, and
+items bit-identical to the same product launched alone (per call: the
+merge-path entry point with the same tile, or plan.execute).
 
 Served: the 64-matrix pool (10k-4M rows, 0.1M-8M nonzeros, Zipf 1.8-2.2)
 resident in HBM, a seeded Poisson stream of requests with 1-32 fp32
@@ -51,8 +53,21 @@ def c4():
     dev = [loops.csr_t.from_host(m.rows, m.cols, m.offsets, m.indices, m.values) for m in mats]
     xs = [torch.empty(m.cols, device="cuda") for m in mats]
     ys = [torch.full((m.rows,), float("nan"), device="cuda") for m in mats]
-    items = [BatchItem(A, x, y, s, i) for A, x, y, (s, i) in zip(dev, xs, ys, picks)]
+    items = [BatchItem(A, x, y, s, i,
+                       plan=loops.spmv_plan(A, layout="blocks", for_batches=True)
+                       if s == "blocks" else None)
+             for A, x, y, (s, i) in zip(dev, xs, ys, picks)]
     return mats, picks, items
+
+
+def test_c4_tuned_picks_use_both_layouts(c4):
+    """The committed table picks row-block plans for the larger matrices
+    and merge-path tiles for the others."""
+    mats, picks, _ = c4
+    kinds = {s for s, _ in picks}
+    assert "blocks" in kinds and "merge_path_flat" in kinds
+    nnz_b = [m.nnz for m, (s, _) in zip(mats, picks) if s == "blocks"]
+    assert min(nnz_b) >= tune.BLOCKS_MIN_NNZ
 
 
 def test_c4_batch_tuned_graph_replay(c4):
@@ -83,8 +98,9 @@ def test_c4_batch_tuned_graph_replay(c4):
 
 def test_c4_items_alone_equal_in_the_mix(c4):
     """A seeded sample of 24 items: the batch result equals the item alone
-    (a batch of one) and, for merge-path items, the per-call entry point
-    with the same tile (256 lanes x items_per_thread steps)."""
+    (a batch of one) and the per-call product: for merge-path items the
+    entry point with the same tile (256 lanes x items_per_thread steps), for
+    plan items plan.execute."""
     mats, picks, items = c4
     for it, i in zip(items, range(len(items))):
         it.x.copy_(torch.from_numpy(synth.uniform(mats[i].cols, seed=7 + i)))
@@ -98,10 +114,13 @@ def test_c4_items_alone_equal_in_the_mix(c4):
         Batch([it]).launch()
         torch.cuda.synchronize()
         assert np.array_equal(it.y.cpu().numpy().view(np.uint32), mixed.view(np.uint32)), i
-        if it.schedule == "merge_path_flat":
+        if it.schedule in ("merge_path_flat", "blocks"):
             y = torch.full_like(it.y, float("nan"))
-            loops.run_spmv("merge_path_flat", it.csr, it.x, y, threads_per_block=256,
-                           items_per_thread=it.items_per_thread or 8)
+            if it.plan is not None:
+                it.plan.execute(it.x, y)
+            else:
+                loops.run_spmv("merge_path_flat", it.csr, it.x, y, threads_per_block=256,
+                               items_per_thread=it.items_per_thread or 8)
             torch.cuda.synchronize()
             assert np.array_equal(y.cpu().numpy().view(np.uint32), mixed.view(np.uint32)), i
 

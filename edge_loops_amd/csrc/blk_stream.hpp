@@ -71,6 +71,16 @@ inline blk_exec_args blk_args_of(const blk_plan_state& st, float* y) {
   return a;
 }
 
+// A load of a value the whole wavefront shares, from plan arrays that no
+// kernel writes while streams run: through the constant address space, so
+// that it is a scalar load (served by the scalar cache, off the vector memory
+// path the x gathers keep busy; a vector load here sits in the round's
+// critical path and cost the C2 stream kernel 15 %).
+template <typename T>
+__device__ __forceinline__ T blk_ld_uniform(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // LDS of one stream's workgroup: the block's y and a round's products.
 struct blk_smem {
   float acc[kBlkRows];
@@ -115,7 +125,7 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
                                                const int* __restrict__ round_off,
                                                const int* __restrict__ round_base, int lane,
                                                int wave, int k0, blk_round_regs<V>& g) {
-  g.n = round_off[r + 1] - round_off[r];
+  g.n = blk_ld_uniform(round_off + r + 1) - blk_ld_uniform(round_off + r);
 #pragma unroll
   for (int j = 0; j < kBlkIpt; ++j) {
     g.pw[j] = 0;
@@ -124,7 +134,7 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
   // The wave's slab exists when its first entry does; positions past the
   // round's end inside it are zero padding.
   if (wave * kBlkSlab < g.n) {
-    const long long s0 = ((long long)round_base[r] + wave) * kBlkSlab;
+    const long long s0 = ((long long)blk_ld_uniform(round_base + r) + wave) * kBlkSlab;
     const long long p = s0 + lane * 4, p2 = p + kBlkSlab / 2;
     const i32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p));
     const i32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p2));
@@ -152,8 +162,9 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
   const int base = __builtin_amdgcn_readfirstlane(wave) * kBlkSlab;
   g.edge_prev = -1;
   g.edge_next = -1;
-  if (base > 0 && base < g.n) g.edge_prev = (int)(rw[(base - 2) >> 1] >> 16);
-  if (base + kBlkSlab < g.n) g.edge_next = (int)(rw[(base + kBlkSlab) >> 1] & 0xffffu);
+  if (base > 0 && base < g.n) g.edge_prev = (int)(blk_ld_uniform(rw + ((base - 2) >> 1)) >> 16);
+  if (base + kBlkSlab < g.n)
+    g.edge_next = (int)(blk_ld_uniform(rw + ((base + kBlkSlab) >> 1)) & 0xffffu);
 }
 
 // Segmented inclusive scan over a wavefront with DPP lane moves: shifts by
@@ -236,23 +247,23 @@ __device__ __forceinline__ void blk_stream(const blk_exec_args& a, const float* 
   float* stage = sm.stage;
   const int S = a.S;
   const int s = piece % S;
-  const int b = a.blk_order[piece / S];
+  const int b = blk_ld_uniform(a.blk_order + piece / S);
   const int q = b * S + s;  // stream
   // stream rows: the compressed list positions rs .. rs + nrows - 1, or the
   // block's rows r0 .. r0 + nrows - 1 (<= kBlkRows either way)
   int r0 = 0, rs = 0, nrows;
   if (a.rlist) {
     const int* rp = a.rstart + (long long)s * (a.nb + 1) + b;
-    rs = rp[0];
-    nrows = rp[1] - rs;
+    rs = blk_ld_uniform(rp);
+    nrows = blk_ld_uniform(rp + 1) - rs;
   } else {
-    r0 = a.blk_start[b];
-    nrows = a.blk_start[b + 1] - r0;
+    r0 = blk_ld_uniform(a.blk_start + b);
+    nrows = blk_ld_uniform(a.blk_start + b + 1) - r0;
   }
   const float* __restrict__ xs = x + (long long)s * a.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int k0 = threadIdx.x * kBlkIpt;
-  const int rb = a.stream_round[q], re = a.stream_round[q + 1];
+  const int rb = blk_ld_uniform(a.stream_round + q), re = blk_ld_uniform(a.stream_round + q + 1);
   blk_round_regs<V> cur;
   if (rb < re)
     blk_load_round<V>(rb, packed, vals, srow, round_off, round_base, lane, wave, k0, cur);

@@ -48,7 +48,9 @@ def _time(fn, reps):
     return e0.elapsed_time(e1) / reps  # ms
 
 
-def batched(count=512, reps=10, table=None, log=print):
+def batched(count=512, reps=10, table=None, log=print, plain=True):
+    """The batched form (config 4).  plain=False skips the per-call legs (a
+    profile of the batch kernels only)."""
     import torch
     from edge_loops_amd import tune
     t0 = time.time()
@@ -132,6 +134,8 @@ def batched(count=512, reps=10, table=None, log=print):
     results["batched_merge8"] = [y.clone() for y in ys]
     del legs
 
+    if not plain:
+        return out
     # plain per-call
     wss = [torch.empty(max(spmv_workspace_size("merge_path_flat", A, threads_per_block=256,
                                                items_per_thread=8), 256),

@@ -283,7 +283,7 @@ def main():
     log(f"cold: {ms_per_step*1e3:.1f} us/step, kernel {kern_avg*1e3:.1f} us, "
         f"{value:.1f} GFLOP/s, {nbytes/kern_avg/1e6:.0f} GB/s")
 
-    extras = {"plan": {"layout": layout, "panels": plans[0].panels,
+    extras = {"plan": {"layout": layout, "panels": plans[0].panels, "blocks": plans[0].blocks,
                        "build_ms": round(build_ms, 2),
                        "device_bytes": plans[0].device_bytes}}
     if not args.quick:

@@ -57,6 +57,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cstdint>
 #include <type_traits>
 #include <vector>
@@ -112,6 +113,41 @@ __device__ __forceinline__ int blk_row_of(const int* __restrict__ offsets, int r
   return lo;
 }
 
+// Calls f(nz, r) for every nonzero nz (row r), entries spread over the grid
+// in workgroup ranges of 256 x kBlkEntryRun: lane l of a workgroup takes
+// entries base + 256 j + l (coalesced), and the rows are found by a search
+// narrowed to the rows of the workgroup's range, continued from the lane's
+// previous row (a few cached steps instead of a search over all rows per
+// nonzero).  blockDim.x must be 256.
+constexpr int kBlkEntryRun = 16;
+template <typename F>
+__device__ __forceinline__ void blk_for_entries(const int* __restrict__ offsets, int rows,
+                                                long long nnz, F&& f) {
+  __shared__ int s_rows[2];
+  const long long per = 256LL * kBlkEntryRun;
+  for (long long base = (long long)blockIdx.x * per; base < nnz;
+       base += (long long)gridDim.x * per) {
+    const long long last = (base + per < nnz ? base + per : nnz) - 1;
+    if (threadIdx.x < 2) s_rows[threadIdx.x] = blk_row_of(offsets, rows, threadIdx.x ? last : base);
+    __syncthreads();
+    int r = s_rows[0];
+    const int hi = s_rows[1] + 1;  // offsets[hi] > last
+    __syncthreads();                // s_rows is rewritten by the next range
+    for (int j = 0; j < kBlkEntryRun; ++j) {
+      const long long nz = base + 256LL * j + threadIdx.x;
+      if (nz > last) break;
+      int lo = r, h = hi;  // offsets[lo] <= nz < offsets[h]
+      while (h - lo > 1) {
+        const int mid = (lo + h) >> 1;
+        if (offsets[mid] <= nz) lo = mid;
+        else h = mid;
+      }
+      r = lo;
+      f(nz, r);
+    }
+  }
+}
+
 __device__ __forceinline__ long long blk_bucket(const blk_geom& g, int b, int col) {
   const int s = col / g.W;
   const int ch = (col - s * g.W) / kBlkChunkCols;
@@ -123,15 +159,14 @@ __global__ __launch_bounds__(256) void blk_mask_kernel(const int* __restrict__ o
                                                        const int* __restrict__ indices,
                                                        blk_geom g,
                                                        unsigned long long* __restrict__ mask) {
-  for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
-       nz += (long long)gridDim.x * blockDim.x) {
-    const int v = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
+  blk_for_entries(offsets, g.rows, g.nnz, [&](long long nz, int row) {
+    const int v = blk_vrow_of(g, offsets, row, nz);
     const unsigned long long bit = 1ull << (indices[nz] / g.W);
     // bits are only ever set: most entries of a long row find theirs set
     // already and skip the atomic
     if (!(__hip_atomic_load(&mask[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
       atomicOr(&mask[v], bit);
-  }
+  });
 }
 
 // fl[s V + v] = bit s of mask[v].
@@ -165,11 +200,10 @@ __global__ __launch_bounds__(256) void blk_rstart_kernel(const int* __restrict__
 __global__ __launch_bounds__(256) void blk_count_kernel(const int* __restrict__ offsets,
                                                         const int* __restrict__ indices,
                                                         blk_geom g, int* __restrict__ cnt) {
-  for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
-       nz += (long long)gridDim.x * blockDim.x) {
-    const int r = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
+  blk_for_entries(offsets, g.rows, g.nnz, [&](long long nz, int row) {
+    const int r = blk_vrow_of(g, offsets, row, nz);
     atomicAdd(&cnt[blk_bucket(g, blk_block_of(g, r), indices[nz])], 1);
-  }
+  });
 }
 
 // Rounds of stream q: its chunks in order, packed greedily into rounds of at
@@ -255,21 +289,20 @@ __global__ __launch_bounds__(256) void blk_scatter_kernel(
     const V* __restrict__ values, blk_geom g, const int* __restrict__ cnt,
     int* __restrict__ cursor, uint32_t* __restrict__ packed, uint16_t* __restrict__ trow,
     V* __restrict__ vals) {
-  for (long long nz = (long long)blockIdx.x * blockDim.x + threadIdx.x; nz < g.nnz;
-       nz += (long long)gridDim.x * blockDim.x) {
-    const int r = blk_vrow_of(g, offsets, blk_row_of(offsets, g.rows, nz), nz);
+  blk_for_entries(offsets, g.rows, g.nnz, [&](long long nz, int row) {
+    const int r = blk_vrow_of(g, offsets, row, nz);
     const int b = blk_block_of(g, r);
     const int col = indices[nz];
     const int sg = col / g.W;
     const long long bucket = blk_bucket(g, b, col);
-    if (cnt[bucket] > kBlkRound) continue;
+    if (cnt[bucket] > kBlkRound) return;
     const int pos = atomicAdd(&cursor[bucket], 1);
     packed[pos] = (uint32_t)(col - sg * g.W);
     trow[pos] = (uint16_t)(g.excl ? g.excl[(long long)sg * g.V + r] -
                                         g.excl[(long long)sg * g.V + g.blk_start[b]]
                                   : r - g.blk_start[b]);
     vals[pos] = values[nz];
-  }
+  });
 }
 
 // big[...] = the buckets of more than kBlkRound entries (in any order; each
@@ -585,10 +618,12 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batche
   if ((e = hipMalloc(&sc0, b_vrow + b_mask + 256)) != hipSuccess) return e;
   int* vrow = P.nsplit > 0 ? reinterpret_cast<int*>(sc0) : nullptr;
   unsigned long long* mask = masks ? reinterpret_cast<unsigned long long*>(sc0 + b_vrow) : nullptr;
-  long long grid = ((long long)A.nnz + 255) / 256;
+  // entry kernels (blk_for_entries): 256 x kBlkEntryRun entries per workgroup range
+  long long grid = ((long long)A.nnz + 256 * kBlkEntryRun - 1) / (256 * kBlkEntryRun);
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
-  std::vector<unsigned long long> mask_h;
+  // filled by the copy below: not value-initialised first
+  std::unique_ptr<unsigned long long[]> mask_h;
   if (vrow)
     e = hipMemcpyAsync(vrow, vrow_h.data(), vrow_h.size() * sizeof(int), hipMemcpyHostToDevice,
                        s);
@@ -599,9 +634,9 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batche
       blk_mask_kernel<<<(unsigned)grid, 256, 0, s>>>(A.offsets, A.indices, g0, mask);
       e = hipGetLastError();
     }
-    mask_h.resize((size_t)V);
+    mask_h.reset(new unsigned long long[(size_t)V]);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(mask_h.data(), mask, (size_t)V * sizeof(unsigned long long),
+      e = hipMemcpyAsync(mask_h.get(), mask, (size_t)V * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }
@@ -610,14 +645,60 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batche
     return e;
   }
 
+  // Segment masks of up to 4 segments as 16-bit lanes: spread[m] has a 1 in
+  // lane s for every bit s of m.
+  unsigned long long spread[16];
+  for (int m = 0; m < 16; ++m) {
+    spread[m] = 0;
+    for (int sg = 0; sg < 4; ++sg)
+      if ((m >> sg) & 1) spread[m] |= 1ull << (16 * sg);
+  }
+  static_assert(kBlkRows < (1 << 15), "row counts fit the 15 low bits of a 16-bit lane");
   // Compressed only where it relaxes the row capacity: when rows have
   // entries in at most 3/4 of the (segment, row) pairs.
   std::vector<long long> rows_in_seg((size_t)P.S, 0);
   long long flagged = 0;
-  for (unsigned long long mm : mask_h)
-    for (; mm; mm &= mm - 1) ++rows_in_seg[__builtin_ctzll(mm)], ++flagged;
+  if (masks && P.S <= 4) {
+    // 16-bit lanes of row counts per segment, one add per row, emptied into
+    // 64-bit totals before a lane can overflow
+    unsigned long long lanes = 0;
+    auto flush = [&] {
+      for (int sg = 0; sg < P.S; ++sg) rows_in_seg[sg] += (long long)((lanes >> (16 * sg)) & 0xffffu);
+      lanes = 0;
+    };
+    for (long long v = 0; v < V; ++v) {
+      lanes += spread[mask_h[v] & 15u];
+      if ((v & 0x7fff) == 0x7fff) flush();
+    }
+    flush();
+  } else if (masks) {
+    for (long long v = 0; v < V; ++v)
+      for (unsigned long long mm = mask_h[v]; mm; mm &= mm - 1) ++rows_in_seg[__builtin_ctzll(mm)];
+  }
+  for (long long c : rows_in_seg) flagged += c;
   const bool comp = masks && flagged * 4 <= 3LL * P.S * V;
+  // Nonzeros of every virtual row (a row cut into m pieces: piece i holds
+  // entries i, i + m, ...), so the cut below divides nothing.
+  std::vector<int> vlen((size_t)V);
+  for (long long r = 0, v = 0; r < A.rows; ++r) {
+    const long long L = off_h[r + 1] - off_h[r];
+    if (L <= cap) {
+      vlen[v++] = (int)L;
+      continue;
+    }
+    const long long m = (L + cap - 1) / cap;
+    for (long long i = 0; i < m; ++i) vlen[v++] = (int)((L - i + m - 1) / m);
+  }
   std::vector<long long> bnnz;
+  // Greedy cut: a virtual row closes the block before it when the block
+  // would exceed its row capacity (compressed: some segment of the row
+  // already holds kBlkRows stream rows) or when stopping short of the
+  // target is closer to it than overshooting.  With up to 4 segments the
+  // per-segment row counts are 16-bit lanes of one word, so the capacity
+  // test is a branch-free zero-lane test (the same cut as counting segment
+  // by segment, 4x faster on the 2M-row matrix).
+  const unsigned long long cap_lanes = spread[15] * (unsigned long long)kBlkRows;
+  const bool packed4 = comp && P.S <= 4;
   std::vector<int> cs((size_t)P.S);
   auto cut = [&](long long want) {
     long long target = A.nnz / want > 0 ? A.nnz / want : 1;
@@ -626,35 +707,42 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batche
       starts.assign(1, 0);
       bnnz.clear();
       std::fill(cs.begin(), cs.end(), 0);
-      int rows_in = 0, v = 0;
+      unsigned long long lanes = 0;
+      int rows_in = 0;
       long long nnz_in = 0;
-      auto add = [&](long long L) {
+      for (long long v = 0; v < V; ++v) {
+        const long long L = vlen[v];
         bool full = rows_in == kBlkRows;
-        if (comp) {
+        if (packed4) {
+          // lanes equal to kBlkRows: the high bit of each zero lane of x
+          const unsigned long long x = lanes ^ cap_lanes;
+          const unsigned long long z =
+              ~(((x & 0x7fff7fff7fff7fffull) + 0x7fff7fff7fff7fffull) | x) & 0x8000800080008000ull;
+          const unsigned at_cap = (unsigned)((z >> 15) & 1) | (unsigned)((z >> 30) & 2) |
+                                  (unsigned)((z >> 45) & 4) | (unsigned)((z >> 60) & 8);
+          full = ((unsigned)mask_h[v] & at_cap) != 0;
+        } else if (comp) {
           full = false;
           for (unsigned long long mm = mask_h[v]; mm; mm &= mm - 1)
             full = full || cs[__builtin_ctzll(mm)] == kBlkRows;
         }
         if (rows_in > 0 &&
             (full || (nnz_in + L > target && nnz_in + L - target > target - nnz_in))) {
-          starts.push_back(v);
+          starts.push_back((int)v);
           bnnz.push_back(nnz_in);
           rows_in = 0;
           nnz_in = 0;
+          lanes = 0;
           std::fill(cs.begin(), cs.end(), 0);
         }
-        if (comp)
+        if (packed4)
+          lanes += spread[mask_h[v] & 15u];
+        else if (comp)
           for (unsigned long long mm = mask_h[v]; mm; mm &= mm - 1) ++cs[__builtin_ctzll(mm)];
         ++rows_in;
         nnz_in += L;
-        ++v;
-      };
-      for (int r = 0; r < A.rows; ++r) {
-        const long long L = off_h[r + 1] - off_h[r];
-        const long long m = L > cap ? (L + cap - 1) / cap : 1;
-        for (long long i = 0; i < m; ++i) add((L - i + m - 1) / m);
       }
-      starts.push_back(v);
+      starts.push_back((int)V);
       bnnz.push_back(nnz_in);
       if ((long long)bnnz.size() <= want || tries == 63) break;
       // the row capacity, not the target, sets the count: stop when it no

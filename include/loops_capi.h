@@ -235,13 +235,14 @@ typedef enum loops_plan_layout_t {
                                    threads_per_block, items_per_thread) is
                                    set, else PANELS.  The rule weighs
                                    execution time only: a BLOCKS plan costs
-                                   1.5-45 ms to build (several stream
+                                   1-28 ms to build (several stream
                                    synchronisations) and about 1.45x the
                                    CSR's device memory, and repays the build
                                    against per-call loops_spmv after roughly
-                                   200 executions (against a PANELS plan:
-                                   350-900).  For a matrix used a few times
-                                   call loops_spmv or ask for PANELS.       */
+                                   115 executions on a 2M-row, 40M-nonzero
+                                   matrix (against a PANELS plan: 250-560).
+                                   For a matrix used a few times call
+                                   loops_spmv or ask for PANELS.            */
   LOOPS_PLAN_LAYOUT_PANELS = 1,
   LOOPS_PLAN_LAYOUT_BLOCKS = 2
 } loops_plan_layout_t;

@@ -153,6 +153,30 @@ def test_work_oriented_tunings(tpb, grid):
     _check(y, off, idx, val, x, tag=f"wo{tpb}x{grid}")
 
 
+@pytest.mark.parametrize("tpb,grid", [(128, 512), (64, 1), (256, 77)])
+@pytest.mark.parametrize("name", ["skewed", "powerlaw_60k", "mostly_empty", "c1"])
+def test_work_oriented_unsplit_rows_bit_exact(name, tpb, grid):
+    """A row that lies inside one thread's share is summed by that thread
+    alone, front to back, so it equals the reference's sequential fp32
+    validator loop bit for bit; rows cut by a share boundary go through the
+    fix-up and meet the usual bar."""
+    off, idx, val, cols = MATS[name]
+    x = np.random.default_rng(17).uniform(-1, 1, cols).astype(np.float32)
+    A = _dev(off, idx, val, cols)
+    y = _run("work_oriented", A, x, threads_per_block=tpb, grid_size=grid)
+    xy = oracle.wo_coords(off, tpb * grid)
+    split = np.zeros(off.shape[0] - 1, bool)
+    sx, sy = xy[:, 0].astype(np.int64), xy[:, 1].astype(np.int64)
+    inside = (sx < split.shape[0]) & (sy > off[np.minimum(sx, split.shape[0] - 1)])
+    split[sx[inside]] = True
+    y32 = oracle.spmv_f32(off, idx, val, x)
+    wpt = -(-(off.shape[0] - 1 + int(off[-1])) // (tpb * grid))  # merge items per share
+    if wpt >= 64:
+        assert (~split).mean() > 0.5, (name, wpt)
+    assert np.array_equal(y[~split].view(np.uint32), y32[~split].view(np.uint32)), name
+    _check(y, off, idx, val, x, tag=f"wo{tpb}x{grid}")
+
+
 @pytest.mark.parametrize("name", ["c1", "skewed", "powerlaw_60k", "one_row", "mostly_empty"])
 def test_partitions_bit_exact(name):
     off, idx, val, cols = MATS[name]

@@ -287,6 +287,35 @@ __global__ __launch_bounds__(TPB) void spmv_group_mapped_kernel(
 // per work-item, grid-stride) and the schedule-free original.  Both sum each
 // row sequentially in CSR order, exactly like the CPU validator.
 // ---------------------------------------------------------------------------
+// Sum over nonzeros b .. e - 1 of value * x[column], added front to back
+// (the order of the reference's CPU validator).  The loads of kSeqBatch
+// nonzeros are all issued before their products are added, so a lane waits
+// on memory once per batch; the operations and their order are unchanged.
+constexpr int kSeqBatch = 16;
+template <typename V>
+__device__ __forceinline__ float seq_row_dot(const int* __restrict__ indices,
+                                             const V* __restrict__ values,
+                                             const float* __restrict__ x, long long b,
+                                             long long e) {
+  float sum = 0.f;
+  for (long long nz = b; nz < e; nz += kSeqBatch) {
+    const int n = e - nz < kSeqBatch ? (int)(e - nz) : kSeqBatch;
+    int col[kSeqBatch];
+    float val[kSeqBatch], xv[kSeqBatch];
+#pragma unroll
+    for (int j = 0; j < kSeqBatch; ++j) {
+      col[j] = j < n ? ld_idx(indices + nz + j) : 0;
+      val[j] = j < n ? value_traits<V>::load1(values + nz + j) : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kSeqBatch; ++j) xv[j] = j < n ? x[col[j]] : 0.f;
+#pragma unroll
+    for (int j = 0; j < kSeqBatch; ++j)
+      if (j < n) sum += val[j] * xv[j];
+  }
+  return sum;
+}
+
 template <typename V>
 __global__ void spmv_thread_mapped_kernel(
     loops::schedule::setup<algorithms_t::thread_mapped, 1, 1, int, int,
@@ -294,10 +323,8 @@ __global__ void spmv_thread_mapped_kernel(
     const int* __restrict__ indices, const V* __restrict__ values,
     const float* __restrict__ x, float* __restrict__ y) {
   for (auto row : config.tiles()) {
-    float sum = 0.f;
-    for (auto nz : config.atoms(row))
-      sum += value_traits<V>::load1(values + nz) * x[ld_idx(indices + nz)];
-    y[row] = sum;
+    const auto atoms = config.atoms(row);  // the row's nonzeros, in CSR order
+    y[row] = seq_row_dot(indices, values, x, *atoms.begin(), *atoms.end());
   }
 }
 
@@ -309,10 +336,7 @@ __global__ void spmv_original_kernel(const int* __restrict__ offsets,
                                      float* __restrict__ y, int rows) {
   for (long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
        row < rows; row += (long long)gridDim.x * blockDim.x) {
-    float sum = 0.f;
-    for (int nz = offsets[row]; nz < offsets[row + 1]; ++nz)
-      sum += value_traits<V>::load1(values + nz) * x[ld_idx(indices + nz)];
-    y[row] = sum;
+    y[row] = seq_row_dot(indices, values, x, offsets[row], offsets[row + 1]);
   }
 }
 

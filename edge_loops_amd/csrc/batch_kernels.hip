@@ -377,9 +377,17 @@ __global__ __launch_bounds__(kBlkTpb) void batch_blocks_kernel(const char* __res
     const batch_item_dev& it = items[item];
     const blk_exec_args& a = blk[it.blk];
     if (sub >= a.nb * a.S) return;
-    if (it.vtype == LOOPS_VALUE_BF16) blk_stream<bf16_t>(a, it.x, sub, sm);
-    else if (it.vtype == LOOPS_VALUE_F16) blk_stream<f16_t>(a, it.x, sub, sm);
-    else blk_stream<float>(a, it.x, sub, sm);
+    const bool ahead = blk_two_ahead(a.S);
+    if (it.vtype == LOOPS_VALUE_BF16) {
+      if (ahead) blk_stream<bf16_t, true>(a, it.x, sub, sm);
+      else blk_stream<bf16_t, false>(a, it.x, sub, sm);
+    } else if (it.vtype == LOOPS_VALUE_F16) {
+      if (ahead) blk_stream<f16_t, true>(a, it.x, sub, sm);
+      else blk_stream<f16_t, false>(a, it.x, sub, sm);
+    } else {
+      if (ahead) blk_stream<float, true>(a, it.x, sub, sm);
+      else blk_stream<float, false>(a, it.x, sub, sm);
+    }
   });
 }
 

@@ -167,6 +167,59 @@ __device__ __forceinline__ void blk_load_round(int r, const uint32_t* __restrict
     g.edge_next = (int)(blk_ld_uniform(rw + ((base + kBlkSlab) >> 1)) & 0xffffu);
 }
 
+// Loads round r of a stream into g; live == false (a round past the
+// stream's end) gives an empty round (n = 0).  Every vector load is issued
+// whatever the round holds, so each round issues the same vector loads and
+// the compiler's wait counts stay exact across the loop (a conditional load,
+// or a select on a loaded value, makes it wait for every load in flight,
+// the prefetched ones included).  Wavefronts and lanes with nothing to load
+// read the plan's first slab and round 0's slot rows instead: valid
+// addresses, shared by all of them (so they hit in cache), holding column
+// words of the plan (so the x reads they lead to stay inside the segment);
+// what they read is never stored or summed (slots past n are not used).
+template <typename V>
+__device__ __forceinline__ void blk_load_round_all(int r, bool live,
+                                               const uint32_t* __restrict__ packed,
+                                               const V* __restrict__ vals,
+                                               const uint16_t* __restrict__ srow,
+                                               const int* __restrict__ round_off,
+                                               const int* __restrict__ round_base, int lane,
+                                               int wave, int k0, blk_round_regs<V>& g) {
+  g.n = live ? blk_ld_uniform(round_off + r + 1) - blk_ld_uniform(round_off + r) : 0;
+  // The wave's slab exists when its first entry does; positions past the
+  // round's end inside it are zero padding.
+  const bool has = wave * kBlkSlab < g.n;
+  const long long s0 = has ? ((long long)blk_ld_uniform(round_base + r) + wave) * kBlkSlab : 0;
+  const long long p = s0 + lane * 4, p2 = p + kBlkSlab / 2;
+  const i32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p));
+  const i32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(packed + p2));
+  g.pw[0] = c0.x; g.pw[1] = c0.y; g.pw[2] = c0.z; g.pw[3] = c0.w;
+  g.pw[4] = c1.x; g.pw[5] = c1.y; g.pw[6] = c1.z; g.pw[7] = c1.w;
+  if constexpr (std::is_same<V, float>::value) {
+    const i32x4 v0 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(vals + p));
+    const i32x4 v1 = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(vals + p2));
+    g.vw[0] = v0.x; g.vw[1] = v0.y; g.vw[2] = v0.z; g.vw[3] = v0.w;
+    g.vw[4] = v1.x; g.vw[5] = v1.y; g.vw[6] = v1.z; g.vw[7] = v1.w;
+  } else {
+    const i32x4 v = __builtin_nontemporal_load(
+        reinterpret_cast<const i32x4*>(vals + s0 + (long long)lane * kBlkIpt));
+    g.vw[0] = g.vw[1] = v.x; g.vw[2] = g.vw[3] = v.y;
+    g.vw[4] = g.vw[5] = v.z; g.vw[6] = g.vw[7] = v.w;
+  }
+  const uint16_t* rs = srow + (k0 < g.n ? (long long)r * kBlkRound + k0 : 0);
+  g.rows = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(rs));
+  // The neighbours' rows inside the wave come from lane shuffles (phase 2);
+  // only the two slots at the wave's edges are loaded, as aligned words at
+  // wave-uniform addresses.
+  const uint32_t* rw = reinterpret_cast<const uint32_t*>(srow + (long long)r * kBlkRound);
+  const int base = __builtin_amdgcn_readfirstlane(wave) * kBlkSlab;
+  g.edge_prev = -1;
+  g.edge_next = -1;
+  if (base > 0 && base < g.n) g.edge_prev = (int)(blk_ld_uniform(rw + ((base - 2) >> 1)) >> 16);
+  if (base + kBlkSlab < g.n)
+    g.edge_next = (int)(blk_ld_uniform(rw + ((base + kBlkSlab) >> 1)) & 0xffffu);
+}
+
 // Segmented inclusive scan over a wavefront with DPP lane moves: shifts by
 // 1, 2, 4 and 8 inside rows of 16 lanes, then each row takes the total of
 // the rows before it (row_bcast 15 into rows 1 and 3, row_bcast 31 into
@@ -235,7 +288,7 @@ __device__ __forceinline__ void blk_block_scan(float& v, int& f, float* s_v, int
 // block blk_order[piece / S], so consecutive pieces take consecutive
 // segments.  Every lane of the workgroup calls it; the caller separates
 // two calls on one workgroup with a barrier (the LDS is reused).
-template <typename V>
+template <typename V, bool kAhead>
 __device__ __forceinline__ void blk_stream(const blk_exec_args& a, const float* __restrict__ x,
                                            int piece, blk_smem& sm) {
   const uint32_t* __restrict__ packed = a.packed;
@@ -264,34 +317,12 @@ __device__ __forceinline__ void blk_stream(const blk_exec_args& a, const float* 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int k0 = threadIdx.x * kBlkIpt;
   const int rb = blk_ld_uniform(a.stream_round + q), re = blk_ld_uniform(a.stream_round + q + 1);
-  blk_round_regs<V> cur;
-  if (rb < re)
-    blk_load_round<V>(rb, packed, vals, srow, round_off, round_base, lane, wave, k0, cur);
-  // the accumulator is cleared while the first round's loads are in flight
-  for (int i = threadIdx.x; i < nrows; i += kBlkTpb) acc[i] = 0.f;
-  for (int r = rb; r < re; ++r) {
+  // Phase 2 of a round (after the barrier that follows its stage writes):
+  // runs of equal rows over slots k0 .. k0 + m - 1.  prev_row / next_row:
+  // the rows of slots k0 - 1 and k0 + kBlkIpt (-1 outside the round), from
+  // the neighbouring lanes or the wave's edge slots.
+  auto phase2 = [&](const blk_round_regs<V>& cur) {
     const int n = cur.n;
-    // Phase 1: products in column order to their slots; the next round's
-    // loads go out behind this round's x reads.
-    float xv[kBlkIpt];
-#pragma unroll
-    for (int j = 0; j < kBlkIpt; ++j) {
-      const int i = wave * 64 * kBlkIpt + j * 64 + lane;
-      xv[j] = i < n ? xs[cur.pw[j] >> kBlkSlotBits] : 0.f;
-    }
-    blk_round_regs<V> nxt;
-    if (r + 1 < re)
-      blk_load_round<V>(r + 1, packed, vals, srow, round_off, round_base, lane, wave, k0, nxt);
-#pragma unroll
-    for (int j = 0; j < kBlkIpt; ++j) {
-      const int i = wave * 64 * kBlkIpt + j * 64 + lane;
-      if (i < n)
-        stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], j) * xv[j];
-    }
-    __syncthreads();
-    // Phase 2: runs of equal rows over slots k0 .. k0 + m - 1.  prev_row /
-    // next_row: the rows of slots k0 - 1 and k0 + kBlkIpt (-1 outside the
-    // round), from the neighbouring lanes or the wave's edge slots.
     const int m = n - k0 < 0 ? 0 : (n - k0 > kBlkIpt ? kBlkIpt : n - k0);
     const int up_row = __shfl_up((int)cur.rows[kBlkIpt - 1], 1, 64);
     const int down_row = __shfl_down((int)cur.rows[0], 1, 64);
@@ -342,7 +373,77 @@ __device__ __forceinline__ void blk_stream(const blk_exec_args& a, const float* 
       if (internal) acc[first_row] += cin + first_sum;
       if (k0 + m >= n || next_row != row) acc[row] += (internal ? 0.f : cin) + v_own;
     }
-    if (r + 1 < re) cur = nxt;
+  };
+  if constexpr (!kAhead) {
+    // One round in flight: a round's x reads and the next round's loads go
+    // out together and are waited for before its stage writes.
+    blk_round_regs<V> cur;
+    if (rb < re)
+      blk_load_round<V>(rb, packed, vals, srow, round_off, round_base, lane, wave, k0, cur);
+    // the accumulator is cleared while the first round's loads are in flight
+    for (int i = threadIdx.x; i < nrows; i += kBlkTpb) acc[i] = 0.f;
+    for (int r = rb; r < re; ++r) {
+      const int n = cur.n;
+      // Phase 1: products in column order to their slots; the next round's
+      // loads go out behind this round's x reads.
+      float xv[kBlkIpt];
+#pragma unroll
+      for (int j = 0; j < kBlkIpt; ++j) {
+        const int i = wave * 64 * kBlkIpt + j * 64 + lane;
+        xv[j] = i < n ? xs[cur.pw[j] >> kBlkSlotBits] : 0.f;
+      }
+      blk_round_regs<V> nxt;
+      if (r + 1 < re)
+        blk_load_round<V>(r + 1, packed, vals, srow, round_off, round_base, lane, wave, k0, nxt);
+#pragma unroll
+      for (int j = 0; j < kBlkIpt; ++j) {
+        const int i = wave * 64 * kBlkIpt + j * 64 + lane;
+        if (i < n)
+          stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], j) * xv[j];
+      }
+      __syncthreads();
+      phase2(cur);
+      if (r + 1 < re) cur = nxt;
+    }
+  } else {
+    // Two rounds in flight: right after a round's stage writes, the next
+    // round's x reads and the loads of the round after it go out, so they
+    // are in flight through this round's phase 2 (every vector load issued
+    // unconditionally, blk_load_round_all, so the waits stay exact).
+    // (entries past a round's end have column words of the plan and read
+    // inside the segment; their products are not stored)
+    auto gather = [&](const blk_round_regs<V>& g, float* xv) {
+#pragma unroll
+      for (int j = 0; j < kBlkIpt; ++j) xv[j] = xs[g.pw[j] >> kBlkSlotBits];
+    };
+    blk_round_regs<V> cur, nxt;
+    float xv[kBlkIpt];
+    if (rb < re)
+      blk_load_round_all<V>(rb, true, packed, vals, srow, round_off, round_base, lane, wave, k0,
+                            cur);
+    for (int i = threadIdx.x; i < nrows; i += kBlkTpb) acc[i] = 0.f;
+    if (rb < re) {
+      gather(cur, xv);
+      blk_load_round_all<V>(rb + 1, rb + 1 < re, packed, vals, srow, round_off, round_base, lane,
+                            wave, k0, nxt);
+    }
+    for (int r = rb; r < re; ++r) {
+      const int n = cur.n;
+#pragma unroll
+      for (int j = 0; j < kBlkIpt; ++j) {
+        const int i = wave * 64 * kBlkIpt + j * 64 + lane;
+        if (i < n)
+          stage[cur.pw[j] & (kBlkRound - 1)] = blk_word_value<V>(cur.vw[j], j) * xv[j];
+      }
+      blk_round_regs<V> nn;
+      gather(nxt, xv);
+      blk_load_round_all<V>(r + 2, r + 2 < re, packed, vals, srow, round_off, round_base, lane,
+                            wave, k0, nn);
+      __syncthreads();
+      phase2(cur);
+      cur = nxt;
+      nxt = nn;
+    }
   }
   __syncthreads();
   float* out = a.partial + (long long)s * a.part_stride;
@@ -355,6 +456,13 @@ __device__ __forceinline__ void blk_stream(const blk_exec_args& a, const float* 
       __builtin_nontemporal_store(acc[i], out + r0 + i);
   }
 }
+
+// Which stream loop runs a plan: two rounds in flight for plans of one
+// column segment (the plans cut for batches), one round otherwise.  The
+// 2M-row matrix's plan (4 segments, x slices held in L2, 20 rounds per
+// stream) ran 129 against 106 us with two rounds in flight; the 512-matrix
+// batch's one-segment plans 0.726 against 0.787 ms per pass.
+__host__ __device__ inline bool blk_two_ahead(int S) { return S == 1; }
 
 // Sum of the S partials of virtual row v in segment order, loads four at a
 // time (S is 1, 4 or a multiple of 8).

@@ -483,11 +483,11 @@ __global__ __launch_bounds__(256) void blk_lane_order_kernel(
 // ---------------------------------------------------------------------------
 
 // Workgroup b * S + s runs segment s of block blk_order[b].
-template <typename V>
+template <typename V, bool kAhead>
 __global__ __launch_bounds__(kBlkTpb) void spmv_blocks_kernel(blk_exec_args a,
                                                               const float* __restrict__ x) {
   __shared__ blk_smem sm;
-  blk_stream<V>(a, x, (int)blockIdx.x, sm);
+  blk_stream<V, kAhead>(a, x, (int)blockIdx.x, sm);
 }
 
 __global__ __launch_bounds__(256) void spmv_blocks_combine_kernel(
@@ -521,8 +521,11 @@ hipError_t blk_scatter(const csr_args& A, const blk_geom& g, const int* cnt, int
 
 template <typename V>
 hipError_t blk_execute(const blk_plan_state& st, const float* x, float* y, hipStream_t s) {
-  spmv_blocks_kernel<V><<<(unsigned)((long long)st.nb * st.S), kBlkTpb, 0, s>>>(
-      blk_args_of(st, y), x);
+  const unsigned grid = (unsigned)((long long)st.nb * st.S);
+  if (blk_two_ahead(st.S))
+    spmv_blocks_kernel<V, true><<<grid, kBlkTpb, 0, s>>>(blk_args_of(st, y), x);
+  else
+    spmv_blocks_kernel<V, false><<<grid, kBlkTpb, 0, s>>>(blk_args_of(st, y), x);
   return hipGetLastError();
 }
 }  // namespace

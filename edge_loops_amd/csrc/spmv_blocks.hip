@@ -80,6 +80,7 @@ struct blk_geom {
   // for dense plans
   const int* excl;
   long long V;                         // virtual rows
+  int run = 1;                         // entry kernels: entries per lane (blk_entry_run)
 };
 
 // Virtual row of entry nz of row r: the row itself, or for a row cut into
@@ -114,17 +115,16 @@ __device__ __forceinline__ int blk_row_of(const int* __restrict__ offsets, int r
 }
 
 // Calls f(nz, r) for every nonzero nz (row r), entries spread over the grid
-// in workgroup ranges of 256 x kBlkEntryRun: lane l of a workgroup takes
-// entries base + 256 j + l (coalesced), and the rows are found by a search
-// narrowed to the rows of the workgroup's range, continued from the lane's
-// previous row (a few cached steps instead of a search over all rows per
-// nonzero).  blockDim.x must be 256.
-constexpr int kBlkEntryRun = 16;
+// in workgroup ranges of 256 x run: lane l of a workgroup takes entries
+// base + 256 j + l (coalesced), and the rows are found by a search narrowed
+// to the rows of the workgroup's range, continued from the lane's previous
+// row (a few cached steps instead of a search over all rows per nonzero).
+// blockDim.x must be 256.
 template <typename F>
 __device__ __forceinline__ void blk_for_entries(const int* __restrict__ offsets, int rows,
-                                                long long nnz, F&& f) {
+                                                long long nnz, int run, F&& f) {
   __shared__ int s_rows[2];
-  const long long per = 256LL * kBlkEntryRun;
+  const long long per = 256LL * run;
   for (long long base = (long long)blockIdx.x * per; base < nnz;
        base += (long long)gridDim.x * per) {
     const long long last = (base + per < nnz ? base + per : nnz) - 1;
@@ -133,7 +133,7 @@ __device__ __forceinline__ void blk_for_entries(const int* __restrict__ offsets,
     int r = s_rows[0];
     const int hi = s_rows[1] + 1;  // offsets[hi] > last
     __syncthreads();                // s_rows is rewritten by the next range
-    for (int j = 0; j < kBlkEntryRun; ++j) {
+    for (int j = 0; j < run; ++j) {
       const long long nz = base + 256LL * j + threadIdx.x;
       if (nz > last) break;
       int lo = r, h = hi;  // offsets[lo] <= nz < offsets[h]
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void blk_mask_kernel(const int* __restrict__ o
                                                        const int* __restrict__ indices,
                                                        blk_geom g,
                                                        unsigned long long* __restrict__ mask) {
-  blk_for_entries(offsets, g.rows, g.nnz, [&](long long nz, int row) {
+  blk_for_entries(offsets, g.rows, g.nnz, g.run, [&](long long nz, int row) {
     const int v = blk_vrow_of(g, offsets, row, nz);
     const unsigned long long bit = 1ull << (indices[nz] / g.W);
     // bits are only ever set: most entries of a long row find theirs set
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void blk_rstart_kernel(const int* __restrict__
 __global__ __launch_bounds__(256) void blk_count_kernel(const int* __restrict__ offsets,
                                                         const int* __restrict__ indices,
                                                         blk_geom g, int* __restrict__ cnt) {
-  blk_for_entries(offsets, g.rows, g.nnz, [&](long long nz, int row) {
+  blk_for_entries(offsets, g.rows, g.nnz, g.run, [&](long long nz, int row) {
     const int r = blk_vrow_of(g, offsets, row, nz);
     atomicAdd(&cnt[blk_bucket(g, blk_block_of(g, r), indices[nz])], 1);
   });
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void blk_scatter_kernel(
     const V* __restrict__ values, blk_geom g, const int* __restrict__ cnt,
     int* __restrict__ cursor, uint32_t* __restrict__ packed, uint16_t* __restrict__ trow,
     V* __restrict__ vals) {
-  blk_for_entries(offsets, g.rows, g.nnz, [&](long long nz, int row) {
+  blk_for_entries(offsets, g.rows, g.nnz, g.run, [&](long long nz, int row) {
     const int r = blk_vrow_of(g, offsets, row, nz);
     const int b = blk_block_of(g, r);
     const int col = indices[nz];
@@ -621,8 +621,11 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batche
   if ((e = hipMalloc(&sc0, b_vrow + b_mask + 256)) != hipSuccess) return e;
   int* vrow = P.nsplit > 0 ? reinterpret_cast<int*>(sc0) : nullptr;
   unsigned long long* mask = masks ? reinterpret_cast<unsigned long long*>(sc0 + b_vrow) : nullptr;
-  // entry kernels (blk_for_entries): 256 x kBlkEntryRun entries per workgroup range
-  long long grid = ((long long)A.nnz + 256 * kBlkEntryRun - 1) / (256 * kBlkEntryRun);
+  // Entry kernels (blk_for_entries): up to 16 entries per lane, as long as
+  // that leaves 2,048 workgroups or more (small matrices take one per lane,
+  // so the whole GPU searches in parallel).
+  const int run = (int)std::min<long long>(16, std::max<long long>(1, (long long)A.nnz / (256LL * 2048)));
+  long long grid = ((long long)A.nnz + 256LL * run - 1) / (256LL * run);
   if (grid > 65536) grid = 65536;
   if (grid < 1) grid = 1;
   // filled by the copy below: not value-initialised first
@@ -632,7 +635,8 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batche
                        s);
   if (masks && e == hipSuccess) {
     e = hipMemsetAsync(mask, 0, (size_t)V * sizeof(unsigned long long), s);
-    const blk_geom g0{A.rows, A.cols, A.nnz, P.S, P.W, 0, 0, nullptr, vrow, nullptr, V};
+    blk_geom g0{A.rows, A.cols, A.nnz, P.S, P.W, 0, 0, nullptr, vrow, nullptr, V};
+    g0.run = run;
     if (e == hipSuccess && A.nnz > 0) {
       blk_mask_kernel<<<(unsigned)grid, 256, 0, s>>>(A.offsets, A.indices, g0, mask);
       e = hipGetLastError();
@@ -843,7 +847,8 @@ hipError_t blk_plan_build(const csr_args& A, blk_plan_state* st, bool for_batche
   int* fl = comp ? reinterpret_cast<int*>(sc + c_fl) : nullptr;
   int* excl = comp ? reinterpret_cast<int*>(sc + c_excl) : nullptr;
   char* work = sc + c_work;
-  const blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart, vrow, excl, V};
+  blk_geom g{A.rows, A.cols, A.nnz, P.S, P.W, nch, P.nb, bstart, vrow, excl, V};
+  g.run = run;
   const unsigned sgrid = (unsigned)((nstreams + 3) / 4 > 0 ? (nstreams + 3) / 4 : 1);  // wave per stream
   auto fail = [&](hipError_t err) {
     (void)hipFree(sc);

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <loops/util/search.hxx>
 
 #include "device_common.hpp"
@@ -221,46 +223,70 @@ __device__ __forceinline__ void spmm_tile(
 
   for (int cb = 0; cb < k; cb += G * CPL) {
     const int c = cb + g * CPL;
-    // 3b. B rows of this range's nonzeros, all loads in flight together
-    cols_t<CPL> bj[IPT];
-    float vj[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      if (j < cnt) {
-        vj[j] = sm.val[yl + j];
-        bj[j] = load_b<CPL>(B, ldb, sm.col[yl + j], c, k);
-      }
-    }
-    // 3c. walk
+    // 3b + 3c: the B rows of this range's nonzeros, all loads in flight
+    // together, then the walk over its steps.
     float acc[CPL], first_acc[CPL];
-#pragma unroll
-    for (int q = 0; q < CPL; ++q) acc[q] = first_acc[q] = 0.f;
     int x = xl, first_row = -1;
-    int row_end = sm.rowend[x];
-    auto close_row = [&]() {
-      if (first_row < 0) {
-        first_row = x;
+    auto load_walk = [&](auto whole) {
+      constexpr bool kWhole = decltype(whole)::value;
+      cols_t<CPL> bj[IPT];
+      float vj[IPT];
 #pragma unroll
-        for (int q = 0; q < CPL; ++q) first_acc[q] = acc[q];
-      } else if (c < k) {
-        store_c<CPL>(C + (long long)(xs + x) * ldc + c, c, k, acc);
+      for (int j = 0; j < IPT; ++j) {
+        if constexpr (kWhole) {
+          // every lane's columns lie inside B's rows and the tile holds
+          // nonzeros: a load per step whatever cnt is (steps past cnt
+          // read the tile's first B row and are not summed), so no load
+          // sits under a branch and none is waited for before the walk
+          const int jj = j < cnt ? yl + j : 0;
+          vj[j] = sm.val[jj];
+          const float* p = B + (long long)sm.col[jj] * ldb + c;
+          if constexpr (CPL == 4) {
+            const f32x4 q = *reinterpret_cast<const f32x4*>(p);
+            bj[j].v[0] = q.x; bj[j].v[1] = q.y; bj[j].v[2] = q.z; bj[j].v[3] = q.w;
+          } else {
+            bj[j].v[0] = *p;
+          }
+        } else if (j < cnt) {
+          vj[j] = sm.val[yl + j];
+          bj[j] = load_b<CPL>(B, ldb, sm.col[yl + j], c, k);
+        }
       }
 #pragma unroll
-      for (int q = 0; q < CPL; ++q) acc[q] = 0.f;
-      ++x;
-      row_end = sm.rowend[x];
+      for (int q = 0; q < CPL; ++q) acc[q] = first_acc[q] = 0.f;
+      int row_end = sm.rowend[x];
+      auto close_row = [&]() {
+        if (first_row < 0) {
+          first_row = x;
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) first_acc[q] = acc[q];
+        } else if (c < k) {
+          store_c<CPL>(C + (long long)(xs + x) * ldc + c, c, k, acc);
+        }
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) acc[q] = 0.f;
+        ++x;
+        row_end = sm.rowend[x];
+      };
+      int yy = yl;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        if (j < cnt) {
+          while (row_end <= yy) close_row();
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
+          ++yy;
+        }
+      }
+      while (x < xl2) close_row();
     };
-    int yy = yl;
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      if (j < cnt) {
-        while (row_end <= yy) close_row();
-#pragma unroll
-        for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
-        ++yy;
-      }
-    }
-    while (x < xl2) close_row();
+    // A wavefront whose lanes all have whole column slices (CPL columns
+    // inside k) takes the branch-free loads: with a load under a branch the
+    // compiler waited for each B row before issuing the next.
+    if (tile_nnz > 0 && __all(c + CPL <= k))
+      load_walk(std::integral_constant<bool, true>{});
+    else
+      load_walk(std::integral_constant<bool, false>{});
 
     // 4. segmented scan over the ranges, per column
     int f = first_row >= 0 ? 1 : 0;

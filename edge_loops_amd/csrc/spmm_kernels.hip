@@ -20,6 +20,8 @@
 //                  finished by a fixed-order fix-up (as for SpMV), per column.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <climits>
 
@@ -141,42 +143,74 @@ __global__ __launch_bounds__(TPB) void spmm_group_mapped_kernel(
 #pragma unroll
       for (int j = 0; j < CPL; ++j) acc[j] = 0.f;
     };
-    int cj[U];
-    float vj[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      cj[j] = a0 + j < a1 ? ld_idx(idx + a0 + j) : 0;
-      vj[j] = a0 + j < a1 ? value_traits<V>::load1(val + a0 + j) : 0.f;
-    }
-    for (int a = a0; a < a1; a += U) {
-      const int n = a1 - a < U ? a1 - a : U;
-      cols_t<CPL> bj[U];
-#pragma unroll
-      for (int j = 0; j < U; ++j)
-        if (j < n) bj[j] = load_b<CPL>(B, ldb, cj[j], c, k);
-      float vcur[U];
-#pragma unroll
-      for (int j = 0; j < U; ++j) vcur[j] = vj[j];
-      // the next atoms' column ids and values, in flight with the B rows
-      const int an = a + U;
+    auto walk = [&](auto whole) {
+      constexpr bool kWhole = decltype(whole)::value;
+      // kWhole: every lane of the wavefront has whole column slices and the
+      // block holds nonzeros, so every load is issued unconditionally
+      // (atoms past a1 read the block's first atom, their B rows are read
+      // and not summed): with a load under a branch or a select on a loaded
+      // value the compiler waited for each B row before the next
+      auto atom = [&](int i) { return kWhole ? (i < a1 ? i : 0) : i; };
+      int cj[U];
+      float vj[U];
 #pragma unroll
       for (int j = 0; j < U; ++j) {
-        cj[j] = an + j < a1 ? ld_idx(idx + an + j) : 0;
-        vj[j] = an + j < a1 ? value_traits<V>::load1(val + an + j) : 0.f;
-      }
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        if (j < n) {
-          while (next_end <= a + j) {
-            finish(cr);
-            ++cr;
-            next_end = s_end[cr + 1];
-          }
-#pragma unroll
-          for (int t = 0; t < CPL; ++t) acc[t] += vcur[j] * bj[j].v[t];
+        if constexpr (kWhole) {
+          cj[j] = ld_idx(idx + atom(a0 + j));
+          vj[j] = value_traits<V>::load1(val + atom(a0 + j));
+        } else {
+          cj[j] = a0 + j < a1 ? ld_idx(idx + a0 + j) : 0;
+          vj[j] = a0 + j < a1 ? value_traits<V>::load1(val + a0 + j) : 0.f;
         }
       }
-    }
+      for (int a = a0; a < a1; a += U) {
+        const int n = a1 - a < U ? a1 - a : U;
+        cols_t<CPL> bj[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          if constexpr (kWhole) {
+            const float* p = B + (long long)cj[j] * ldb + c;
+            if constexpr (CPL == 4) {
+              const f32x4 qv = *reinterpret_cast<const f32x4*>(p);
+              bj[j].v[0] = qv.x; bj[j].v[1] = qv.y; bj[j].v[2] = qv.z; bj[j].v[3] = qv.w;
+            } else {
+              bj[j].v[0] = *p;
+            }
+          } else if (j < n) {
+            bj[j] = load_b<CPL>(B, ldb, cj[j], c, k);
+          }
+        }
+        float vcur[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) vcur[j] = vj[j];
+        // the next atoms' column ids and values, in flight with the B rows
+        const int an = a + U;
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          if constexpr (kWhole) {
+            cj[j] = ld_idx(idx + atom(an + j));
+            vj[j] = value_traits<V>::load1(val + atom(an + j));
+          } else {
+            cj[j] = an + j < a1 ? ld_idx(idx + an + j) : 0;
+            vj[j] = an + j < a1 ? value_traits<V>::load1(val + an + j) : 0.f;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          if (j < n) {
+            while (next_end <= a + j) {
+              finish(cr);
+              ++cr;
+              next_end = s_end[cr + 1];
+            }
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) acc[t] += vcur[j] * bj[j].v[t];
+          }
+        }
+      }
+    };
+    if (agg > 0 && __all(c + CPL <= k)) walk(std::integral_constant<bool, true>{});
+    else walk(std::integral_constant<bool, false>{});
     while (cr < rhi) {
       finish(cr);
       ++cr;

@@ -43,11 +43,7 @@ struct csr_t {
     coo_t<index_t, value_t, memory_space_t::host> c(coo);
     c.sort_by_row();
     vector_t<offset_t, memory_space_t::host> off(rows + 1);
-    std::size_t p = 0;
-    for (std::size_t r = 0; r <= rows; ++r) {
-      while (p < c.nnzs && std::size_t(c.row_indices[p]) < r) ++p;
-      off[r] = offset_t(p);
-    }
+    detail::indices_to_offsets(c.row_indices, off);
     offsets = vector_t<offset_t, space>(off);
     indices = vector_t<index_t, space>(c.col_indices);
     values = vector_t<value_t, space>(c.values);

@@ -7,5 +7,6 @@
 
 #include <loops/container/coo.hxx>
 #include <loops/container/coordinate.hxx>
+#include <loops/container/csc.hxx>
 #include <loops/container/csr.hxx>
 #include <loops/container/vector.hxx>

@@ -20,27 +20,13 @@
 
 #include <loops/container/coo.hxx>
 #include <loops/error.hxx>
+#include <loops/util/filepath.hxx>
 
 namespace loops {
 
 enum matrix_market_format_t { coordinate, array };
 enum matrix_market_data_t { real, complex, pattern, integer };
 enum matrix_market_storage_scheme_t { general, hermitian, symmetric, skew_symmetric };
-
-/// "dir/name.mtx" -> "name.mtx" (util/filepath.hxx).
-inline std::string extract_filename(const std::string& path) {
-  std::size_t s = path.find_last_of("/\\");
-  return s == std::string::npos ? path : path.substr(s + 1);
-}
-/// "name.mtx" -> "name".
-inline std::string extract_dataset(const std::string& filename) {
-  std::size_t d = filename.find_last_of('.');
-  return d == std::string::npos ? filename : filename.substr(0, d);
-}
-inline bool is_market(const std::string& filename) {
-  return filename.size() > 4 &&
-         filename.compare(filename.size() - 4, 4, ".mtx") == 0;
-}
 
 template <typename index_t, typename offset_t, typename type_t>
 struct matrix_market_t {

@@ -77,3 +77,14 @@ def test_schedule_api_user_kernels(skewed_mtx, tmp_path):
 def test_spmm_cli_runs(alg):
     out = _run([os.path.join(BIN, f"loops.spmm.{alg}"), "-m", CHES, "-n", "10"])
     assert "Elapsed (ms):" in out
+
+
+def test_containers_device():
+    """The container utilities on device memory: the sample matrix, the
+    conversions, COO / CSC / CSR built into device vectors, the device fill
+    kernel (equal to the host fill of the same seed) and random CSR
+    matrices in device memory."""
+    out = _run([os.path.join(BIN, "loops.containers"), "--device"])
+    for group in ("uniform fills (device): ok", "random CSR (device): ok"):
+        assert group in out, out
+    assert out.count("sample, conversions, COO / CSC / CSR: ok") == 2, out

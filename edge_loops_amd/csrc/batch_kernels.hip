@@ -547,7 +547,7 @@ loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
   const loops_csr_view_t& A = b.A;
   if (A.rows < 0 || A.cols < 0 || A.nnzs < 0 || b.k < 1 || b.k > (1 << 20))
     return LOOPS_ERROR_INVALID_VALUE;
-  if (A.rows + A.nnzs >= (int64_t)INT32_MAX || A.cols >= (int64_t)INT32_MAX)
+  if (A.rows + A.nnzs > LOOPS_MAX_ROWS_PLUS_NNZS || A.cols >= (int64_t)INT32_MAX)
     return LOOPS_ERROR_OVERFLOW;
   if (A.value_type != LOOPS_VALUE_F32 && A.value_type != LOOPS_VALUE_BF16 &&
       A.value_type != LOOPS_VALUE_F16)

@@ -39,7 +39,7 @@ constexpr int kRefBlock = 128;  // reference host entries use 128-thread blocks
 loops_status_t check_csr(const loops_csr_view_t* A, csr_args* out) {
   if (!A) return LOOPS_ERROR_INVALID_VALUE;
   if (A->rows < 0 || A->cols < 0 || A->nnzs < 0) return LOOPS_ERROR_INVALID_VALUE;
-  if (A->rows + A->nnzs >= (int64_t)INT_MAX || A->cols >= (int64_t)INT_MAX)
+  if (A->rows + A->nnzs > LOOPS_MAX_ROWS_PLUS_NNZS || A->cols >= (int64_t)INT_MAX)
     return LOOPS_ERROR_OVERFLOW;
   if (A->value_type != LOOPS_VALUE_F32 && A->value_type != LOOPS_VALUE_BF16 &&
       A->value_type != LOOPS_VALUE_F16)
@@ -209,7 +209,7 @@ const char* loops_status_string(loops_status_t status) {
   switch (status) {
     case LOOPS_SUCCESS: return "success";
     case LOOPS_ERROR_INVALID_VALUE: return "invalid value";
-    case LOOPS_ERROR_OVERFLOW: return "index overflow (rows + nnzs must be < 2^31)";
+    case LOOPS_ERROR_OVERFLOW: return "index overflow (rows + nnzs must be at most 2^31 - 1 - 2^20)";
     case LOOPS_ERROR_WORKSPACE: return "workspace missing or too small";
     case LOOPS_ERROR_NOT_SUPPORTED: return "not supported";
     case LOOPS_ERROR_HIP: return "HIP runtime error";
@@ -416,7 +416,7 @@ loops_status_t loops_merge_path_num_tiles(int64_t rows, int64_t nnzs,
                                           int64_t* num_tiles) {
   if (!num_tiles || rows < 0 || nnzs < 0 || items_per_tile <= 0)
     return LOOPS_ERROR_INVALID_VALUE;
-  if (rows + nnzs >= (int64_t)INT_MAX) return LOOPS_ERROR_OVERFLOW;
+  if (rows + nnzs > LOOPS_MAX_ROWS_PLUS_NNZS) return LOOPS_ERROR_OVERFLOW;
   *num_tiles = (rows + nnzs + items_per_tile - 1) / items_per_tile;
   return LOOPS_SUCCESS;
 }
@@ -441,7 +441,7 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
   if (rows < 0 || nnzs < 0 || num_threads <= 0 || !coords ||
       (rows > 0 && !offsets))
     return LOOPS_ERROR_INVALID_VALUE;
-  if (rows + nnzs >= (int64_t)INT_MAX) return LOOPS_ERROR_OVERFLOW;
+  if (rows + nnzs > LOOPS_MAX_ROWS_PLUS_NNZS) return LOOPS_ERROR_OVERFLOW;
   return hip_status(launch_wo_partition(offsets, (int)rows, (int)nnzs,
                                         num_threads, coords, stream));
 }

@@ -27,7 +27,9 @@
  *   - Results are deterministic: no schedule uses floating-point atomics.
  *     Rows split across work units are combined by a fixed-order fix-up.
  *   - Index and offset types are int32, as in every reference example
- *     (examples/spmv/merge_path.cu:18-20); rows + nnzs must be < 2^31.
+ *     (examples/spmv/merge_path.cu:18-20); rows + nnzs must be at most
+ *     LOOPS_MAX_ROWS_PLUS_NNZS, 2^31 - 1 less 2^20: the kernels index
+ *     entries in int32 up to a tile past the last one.
  */
 #ifndef LOOPS_CAPI_H_
 #define LOOPS_CAPI_H_
@@ -43,11 +45,15 @@ extern "C" {
 
 #define LOOPS_CAPI_VERSION 1
 
+/* Largest rows + nnzs of a matrix (int32 offsets and entry positions, with
+ * headroom for the entries a merge tile reads past its end). */
+#define LOOPS_MAX_ROWS_PLUS_NNZS (2147483647LL - (1LL << 20))
+
 /* error.hxx:15-45 (exception_t / throw_if_exception) become status codes. */
 typedef enum loops_status_t {
   LOOPS_SUCCESS = 0,
   LOOPS_ERROR_INVALID_VALUE = 1,   /* null pointer, negative size, bad enum */
-  LOOPS_ERROR_OVERFLOW = 2,        /* rows + nnzs >= 2^31 (int32 offsets)   */
+  LOOPS_ERROR_OVERFLOW = 2,        /* rows + nnzs > LOOPS_MAX_ROWS_PLUS_NNZS */
   LOOPS_ERROR_WORKSPACE = 3,       /* workspace missing or too small        */
   LOOPS_ERROR_NOT_SUPPORTED = 4,   /* schedule / tuning not available       */
   LOOPS_ERROR_HIP = 5,             /* a HIP runtime call failed             */

@@ -49,12 +49,21 @@ def test_num_tiles_and_validation():
     assert L.loops_merge_path_num_tiles(-1, 0, 640, ctypes.byref(T)) == _capi.ERROR_INVALID_VALUE
     assert L.loops_merge_path_num_tiles(1, 0, 0, ctypes.byref(T)) == _capi.ERROR_INVALID_VALUE
     assert L.loops_merge_path_num_tiles(2**30, 2**30, 640, ctypes.byref(T)) == _capi.ERROR_OVERFLOW
+    # the limit, 2^31 - 1 - 2^20 rows + nonzeros, is accepted; one more is not
+    limit = 2**31 - 1 - 2**20
+    assert L.loops_merge_path_num_tiles(4, limit - 4, 2048, ctypes.byref(T)) == 0
+    assert T.value == -(-limit // 2048)
+    assert L.loops_merge_path_num_tiles(4, limit - 3, 2048, ctypes.byref(T)) == _capi.ERROR_OVERFLOW
+    # (non-null stand-in pointers: the size check comes before any launch)
+    assert L.loops_work_oriented_partition(1, 4, limit - 3, 64, 1, None) == _capi.ERROR_OVERFLOW
     # partition / workspace queries reject bad views before touching a GPU
     assert L.loops_merge_path_partition(None, 10, 0, 640, None, None) == _capi.ERROR_INVALID_VALUE
     v = _capi.CsrView(-1, 4, 0, None, None, None, 0)
     n = ctypes.c_size_t()
     assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_INVALID_VALUE
     v = _capi.CsrView(2**30, 4, 2**30, 1, 1, 1, 0)
+    assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_OVERFLOW
+    v = _capi.CsrView(4, 4, limit - 3, 1, 1, 1, 0)
     assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_OVERFLOW
     v = _capi.CsrView(4, 4, 4, 1, 1, 1, 0)
     assert L.loops_spmv_workspace_size(None, 4, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_NOT_SUPPORTED

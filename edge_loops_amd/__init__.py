@@ -7,7 +7,7 @@ result is computed by the library's HIP kernels.
 """
 from . import _capi, algorithms
 from ._capi import LoopsError, lib
-from .algorithms import (merge_path_partition, run_spmm, run_spmv, run_spmv_typed, spmm,
+from .algorithms import (csr_check, merge_path_partition, run_spmm, run_spmv, run_spmv_typed, spmm,
                          spmm_workspace_size, spmv, spmv_plan, spmv_typed_workspace_size,
                          spmv_workspace_size, work_oriented_partition, device_cu_count)
 from .containers import csr_t, matrix_t, vector_t
@@ -17,4 +17,5 @@ __all__ = [
     "run_spmv", "run_spmm", "run_spmv_typed", "spmv_workspace_size", "spmm_workspace_size",
     "spmv_typed_workspace_size",
     "spmv_plan", "merge_path_partition", "work_oriented_partition", "device_cu_count",
+    "csr_check",
 ]

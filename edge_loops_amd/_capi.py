@@ -198,6 +198,7 @@ SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_size_t)]),
     "loops_spmv_typed": (_st, [_vp, _st, ctypes.POINTER(CsrView), _vp, _vp,
                                ctypes.POINTER(Params), _vp, ctypes.c_size_t, _vp]),
+    "loops_csr_check": (_st, [ctypes.POINTER(CsrView), ctypes.POINTER(ctypes.c_uint32), _vp]),
     "loops_spmv_noalloc": (_st, [_vp, _st, ctypes.POINTER(CsrView), _vp, _vp,
                                  ctypes.POINTER(Params), _vp]),
     "loops_merge_path_num_tiles": (_st, [_i64, _i64, _i64, ctypes.POINTER(_i64)]),

@@ -312,6 +312,24 @@ def work_oriented_partition(csr, num_threads, stream=None):
     return coords
 
 
+CSR_FLAGS = {"bad_offsets": 1, "column_range": 2, "unsorted": 4}
+
+
+def csr_check(csr, stream=None):
+    """Structure of a device CSR matrix (loops_csr_check): the names of the
+    LOOPS_CSR_* problems found, an empty list when it is well formed
+    (offsets from 0 to nnz without decreasing, column ids in [0, cols) and
+    ascending inside each row).  Plans need all three; loops_spmv_plan_create
+    rejects a matrix that fails."""
+    _v.check_csr(csr)
+    _v.check_devices(csr)
+    v = csr.view()
+    f = ctypes.c_uint32()
+    _capi.check(_capi.lib().loops_csr_check(ctypes.byref(v), ctypes.byref(f), _stream(stream)),
+                "loops_csr_check")
+    return [name for name, bit in CSR_FLAGS.items() if f.value & bit]
+
+
 def device_cu_count():
     n = ctypes.c_int32()
     _capi.check(_capi.lib().loops_device_cu_count(_capi.handle(), ctypes.byref(n)),

@@ -446,6 +446,19 @@ loops_status_t loops_work_oriented_partition(const int32_t* offsets,
                                         num_threads, coords, stream));
 }
 
+loops_status_t loops_csr_check(const loops_csr_view_t* A, uint32_t* flags,
+                               hipStream_t stream) {
+  if (!flags) return LOOPS_ERROR_INVALID_VALUE;
+  csr_args a;
+  const loops_status_t st = check_csr(A, &a);
+  if (st != LOOPS_SUCCESS) return st;
+  unsigned f = 0;
+  const hipError_t e = csr_structure_flags(a, stream, &f);
+  if (e != hipSuccess) return hip_status(e);
+  *flags = f;
+  return LOOPS_SUCCESS;
+}
+
 loops_status_t loops_spmv_plan_create(loops_handle_t handle,
                                       const loops_csr_view_t* A,
                                       const loops_plan_params_t* params,
@@ -465,6 +478,14 @@ loops_status_t loops_spmv_plan_create(loops_handle_t handle,
   const bool panel_param = p->panels != 0 || p->threads_per_block != 0 || p->items_per_thread != 0;
   if (p->for_batches && (p->layout == LOOPS_PLAN_LAYOUT_PANELS || panel_param))
     return LOOPS_ERROR_INVALID_VALUE;
+  // plans are validated up to this size (larger builds faulted the GPU)
+  if ((long long)a.rows + a.nnz > LOOPS_MAX_PLAN_ROWS_PLUS_NNZS) return LOOPS_ERROR_NOT_SUPPORTED;
+  // a plan bins entries by column and walks rows in column order: reject a
+  // malformed CSR before building anything from it
+  unsigned flags = 0;
+  hipError_t ce = csr_structure_flags(a, stream, &flags);
+  if (ce != hipSuccess) return hip_status(ce);
+  if (flags) return LOOPS_ERROR_INVALID_VALUE;
   int layout = p->layout;
   // AUTO: row blocks from 2^12 columns and 2^19 nonzeros up, unless a
   // PANELS parameter is set.  Blocks against panels: 200k x 200k, 2M

@@ -182,6 +182,8 @@ hipError_t plan_build(const csr_args& A, int P, int tpb, int ipt, plan_state* st
 hipError_t plan_execute(plan_state* st, const float* x, float* y, hipEvent_t ev0,
                         hipEvent_t ev1, hipStream_t s);
 void plan_release(plan_state* st);
+// OR of LOOPS_CSR_* flags of A (0: well formed); synchronises s.
+hipError_t csr_structure_flags(const csr_args& A, hipStream_t s, unsigned* out);
 size_t exclusive_scan_scratch_bytes(long long n);
 hipError_t launch_exclusive_scan(const int* cnt, long long n, int* out, void* scratch,
                                  hipStream_t s);

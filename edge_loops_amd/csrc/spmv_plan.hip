@@ -483,4 +483,74 @@ void plan_release(plan_state* st) {
   st->offsets_mem = st->data_mem = st->aux_mem = nullptr;
 }
 
+// ---------------------------------------------------------------------------
+// CSR structure check (loops_csr_check, and the first step of every plan
+// build): offsets from 0 to nnz without decreasing, column ids in
+// [0, cols), and ascending inside each row.  Work-item i checks row i and
+// entry i; a descent from entry i to entry i + 1 is only allowed where
+// i + 1 starts a row, found by a lower_bound over offsets[0 .. rows].  Every
+// read is inside offsets[0 .. rows] and indices[0 .. nnz), whatever the
+// values are.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void csr_check_kernel(const int* __restrict__ offsets,
+                                                        const int* __restrict__ indices,
+                                                        int rows, int nnz, int cols,
+                                                        unsigned* __restrict__ flags) {
+  unsigned f = 0;
+  const long long n = rows > nnz ? rows : nnz;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < rows) {
+      if (offsets[i] > offsets[i + 1]) f |= LOOPS_CSR_BAD_OFFSETS;
+      if (i == 0 && offsets[0] != 0) f |= LOOPS_CSR_BAD_OFFSETS;
+      if (i == rows - 1 && offsets[rows] != nnz) f |= LOOPS_CSR_BAD_OFFSETS;
+    }
+    if (i < nnz) {
+      const int c = indices[i];
+      if (c < 0 || c >= cols) f |= LOOPS_CSR_COLUMN_RANGE;
+      if (i + 1 < nnz && c > indices[i + 1]) {
+        const long long next = i + 1;
+        int lo = 0, hi = rows + 1;  // first r with offsets[r] >= next
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (offsets[mid] < next) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo > rows || offsets[lo] != next) f |= LOOPS_CSR_UNSORTED;
+      }
+    }
+  }
+  const unsigned long long any = __ballot(f != 0);
+  if (any) {
+    unsigned w = f;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) w |= __shfl_xor(w, m, 64);
+    if ((threadIdx.x & 63) == 0) atomicOr(flags, w);
+  }
+}
+
+hipError_t csr_structure_flags(const csr_args& A, hipStream_t s, unsigned* out) {
+  *out = 0;
+  if (A.rows == 0) {
+    if (A.nnz != 0) *out = LOOPS_CSR_BAD_OFFSETS;
+    return hipSuccess;
+  }
+  unsigned* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(unsigned));
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(d, 0, sizeof(unsigned), s);
+  if (e == hipSuccess) {
+    const long long n = A.rows > A.nnz ? A.rows : A.nnz;
+    long long grid = (n + 255) / 256;
+    if (grid > 16384) grid = 16384;
+    csr_check_kernel<<<(unsigned)grid, 256, 0, s>>>(A.offsets, A.indices, A.rows, A.nnz,
+                                                     A.cols, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  return e;
+}
+
 }  // namespace loops_mi355x

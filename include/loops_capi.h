@@ -271,11 +271,37 @@ typedef struct loops_plan_params_t {
                                 parameter.                                  */
 } loops_plan_params_t;
 
+/* A plan reorders A's entries by column range and so needs a well-formed
+ * CSR (loops_csr_check): it checks A first (one pass over it) and returns
+ * LOOPS_ERROR_INVALID_VALUE for a matrix that is not, before building
+ * anything.  Matrices with rows + nnzs above LOOPS_MAX_PLAN_ROWS_PLUS_NNZS
+ * get LOOPS_ERROR_NOT_SUPPORTED before any device work: plans are
+ * validated on matrices of up to 42M rows + nonzeros, and building plans of
+ * about 2^31 faulted the GPU with no cause found; loops_spmv (every
+ * schedule) is validated up to LOOPS_MAX_ROWS_PLUS_NNZS.  Synchronises the
+ * stream. */
+#define LOOPS_MAX_PLAN_ROWS_PLUS_NNZS (1LL << 26)
 loops_status_t loops_spmv_plan_create(loops_handle_t handle,
                                       const loops_csr_view_t* A,
                                       const loops_plan_params_t* params,
                                       hipStream_t stream,
                                       loops_spmv_plan_t* plan);
+
+/* Structure of a CSR matrix, for callers that want to validate one before
+ * use (the reference's containers assume it; its CSR from COO sorts by
+ * row and column, csr.hxx:86-94).  *flags = 0 when A is well formed, else
+ * an OR of: LOOPS_CSR_BAD_OFFSETS (offsets[0] != 0, offsets[rows] != nnzs,
+ * or decreasing), LOOPS_CSR_COLUMN_RANGE (a column id outside [0, cols)),
+ * LOOPS_CSR_UNSORTED (column ids descending inside a row; repeats are
+ * allowed).  The per-call entry points read columns in row order and need
+ * only offsets and column range to be right; plans need all three.  Reads
+ * offsets[0 .. rows] and indices[0 .. nnzs) only.  Synchronises the
+ * stream; allocates 4 bytes of device memory. */
+#define LOOPS_CSR_BAD_OFFSETS 1u
+#define LOOPS_CSR_COLUMN_RANGE 2u
+#define LOOPS_CSR_UNSORTED 4u
+loops_status_t loops_csr_check(const loops_csr_view_t* A, uint32_t* flags,
+                               hipStream_t stream);
 /* y = A * x (y overwritten); events (optional) around the main kernel.
  * Executions of one plan must not overlap in time (two streams, or graph
  * replays on several streams): they share the plan's partial sums (BLOCKS)

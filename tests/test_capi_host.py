@@ -65,6 +65,11 @@ def test_num_tiles_and_validation():
     assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_OVERFLOW
     v = _capi.CsrView(4, 4, limit - 3, 1, 1, 1, 0)
     assert L.loops_spmv_workspace_size(None, 0, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_OVERFLOW
+    # plans are refused above 2^26 rows + nonzeros, before any device work
+    plan = ctypes.c_void_p()
+    v = _capi.CsrView(4, 4, 2**26 - 3, 1, 1, 1, 0)
+    assert L.loops_spmv_plan_create(None, ctypes.byref(v), None, None,
+                                    ctypes.byref(plan)) == _capi.ERROR_NOT_SUPPORTED
     v = _capi.CsrView(4, 4, 4, 1, 1, 1, 0)
     assert L.loops_spmv_workspace_size(None, 4, ctypes.byref(v), None, ctypes.byref(n)) == _capi.ERROR_NOT_SUPPORTED
     p = _capi.Params(128, 7, 0, 0, None, None, None)  # no (128, 7) tuning

@@ -510,9 +510,11 @@ __global__ __launch_bounds__(256) void csr_check_kernel(const int* __restrict__ 
       if (c < 0 || c >= cols) f |= LOOPS_CSR_COLUMN_RANGE;
       if (i + 1 < nnz && c > indices[i + 1]) {
         const long long next = i + 1;
-        int lo = 0, hi = rows + 1;  // first r with offsets[r] >= next
+        // first r with offsets[r] >= next (64-bit: lo + hi exceeds int
+        // from 2^30 rows)
+        long long lo = 0, hi = (long long)rows + 1;
         while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
+          const long long mid = (lo + hi) >> 1;
           if (offsets[mid] < next) lo = mid + 1;
           else hi = mid;
         }

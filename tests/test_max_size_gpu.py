@@ -134,6 +134,59 @@ def test_max_size_four_rows(x):
     torch.cuda.empty_cache()
 
 
+def test_rows_past_2_30(x):
+    """2^30 + 2^21 rows, 96 nonzeros in the last rows (row ids past 2^30, so
+    the sum of two row ids exceeds int32 in a binary search over rows): the
+    five per-call schedules and the allocation-free entry point give the
+    exact counts, and loops_csr_check, whose search for the row of each
+    column descent runs over all rows, finds the descents at row starts
+    well formed and one inside a row unsorted."""
+    rows = 2**30 + 2**21
+    nzrows = rows - 300 + 3 * np.arange(48)  # every third of the last 300 rows
+    lens = 1 + np.arange(48) % 3
+    cols_of = [(k % 4) + 2 * np.arange(L) for k, L in enumerate(lens)]  # descents at row starts
+    idx = np.concatenate(cols_of).astype(np.int32)
+    nnz = idx.size
+    cum = np.cumsum(lens)
+    offsets = torch.zeros(rows + 1, dtype=torch.int32, device="cuda")
+    for i, r in enumerate(nzrows):
+        hi = nzrows[i + 1] + 1 if i + 1 < len(nzrows) else rows + 1
+        offsets[int(r) + 1:int(hi)] = int(cum[i])
+    assert int(offsets[-1]) == nnz
+    print(f"rows past 2^30: {rows} rows, {nnz} nonzeros", flush=True)
+    A = loops.csr_t(rows, COLS, offsets, torch.from_numpy(idx).cuda(),
+                    torch.ones(nnz, dtype=torch.float32, device="cuda"))
+    assert loops.csr_check(A) == []
+    want = torch.zeros(rows, dtype=torch.float32, device="cuda")
+    want[torch.from_numpy(nzrows).cuda()] = torch.tensor(
+        [float((c % 2).sum()) for c in cols_of], device="cuda")
+    y = torch.empty(rows, dtype=torch.float32, device="cuda")
+    for sched in ("merge_path_flat", "work_oriented", "group_mapped", "thread_mapped",
+                  "original"):
+        y.fill_(float("nan"))
+        getattr(loops.spmv, sched)(A, x, y)
+        torch.cuda.synchronize()
+        assert torch.equal(y, want), sched
+    y.fill_(float("nan"))
+    v, p = A.view(), make_params()
+    _capi.check(_capi.lib().loops_spmv_noalloc(
+        _capi.handle(), _capi.SCHEDULES["merge_path_flat"], ctypes.byref(v), x.data_ptr(),
+        y.data_ptr(), ctypes.byref(p), torch.cuda.current_stream().cuda_stream),
+        "loops_spmv_noalloc")
+    torch.cuda.synchronize()
+    assert torch.equal(y, want), "noalloc"
+    _plans_refused(A)
+    # a descent inside the last row of three entries
+    last3 = int(np.flatnonzero(lens == 3)[-1])
+    bad = idx.copy()
+    a = int(cum[last3]) - 3
+    bad[a], bad[a + 1] = bad[a + 1], bad[a]
+    B = loops.csr_t(rows, COLS, offsets, torch.from_numpy(bad).cuda(), A.values)
+    assert loops.csr_check(B) == ["unsorted"]
+    del A, B, offsets, want, y
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.skip(reason=PLAN_FAULT)
 def test_max_size_plans_many_rows(x):
     rows, length = 2**20 - 1, 2046

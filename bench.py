@@ -24,8 +24,9 @@ per-call drop-in (loops_spmv, no plan) are reported beside it.
 Prints one JSON line (value = GFLOP/s = 2 nnz / step time) with a roofline
 object for the main kernel (algorithmic bytes / its average duration from
 HIP events recorded around it on its stream), the CPU baseline (the
-oracle's sequential validator loop on the host), the device's properties
-and measured HBM ceilings (a 1 GiB copy and read).
+oracle's sequential validator loop on the host), the device's properties,
+measured HBM ceilings (2 GiB streamed and copied by probe kernels) and the
+CSR-order gather ceiling of this matrix beside the per-call merge-path leg.
 """
 import argparse
 import json
@@ -156,6 +157,57 @@ def hbm_ceilings(stream):
     return out
 
 
+def csr_gather_ceiling(copies, x, stream, reps=10):
+    """How fast this GPU reads a CSR matrix's nonzeros in CSR order: the
+    column ids and values streamed (16-byte non-temporal loads) and x
+    gathered at every id, without the row walk or y (hbm_probe_gather),
+    rotating over the copies of A like the timed legs; the best of 1-16
+    workgroups per CU and 1-8 pairs of loads in flight per lane.  Any SpMV
+    that reads A in CSR order does at least this per nonzero."""
+    import ctypes
+
+    import torch
+
+    from edge_loops_amd.hip import Event
+    lib = ctypes.CDLL(os.path.join(ROOT, "edge_loops_amd", "lib", "libloops_hbm.so"))
+    lib.hbm_probe_gather.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p]
+    sink = torch.zeros(256, dtype=torch.int32, device="cuda")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    for A in copies:
+        assert A.indices.dtype == torch.int32 and A.values.dtype == torch.float32
+        assert A.indices.data_ptr() % 16 == 0 and A.values.data_ptr() % 16 == 0
+        assert x.numel() >= A.cols
+
+    def run(i, g, depth):
+        A = copies[i % len(copies)]
+        rc = lib.hbm_probe_gather(A.indices.data_ptr(), A.values.data_ptr(), A.nnzs,
+                                  x.data_ptr(), g, depth, sink.data_ptr(), stream)
+        if rc != 0:
+            raise RuntimeError(f"gather probe failed: hipError {rc}")
+
+    best = (float("inf"), 0, 0)
+    for depth in (1, 2, 4, 8):
+        for per_cu in (1, 2, 4, 8, 16):
+            g = cus * per_cu
+            for i in range(2):
+                run(i, g, depth)
+            torch.cuda.synchronize()
+            e0, e1 = Event(), Event()
+            e0.record(stream)
+            for i in range(reps):
+                run(i, g, depth)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            best = min(best, (e0.elapsed_ms(e1) / reps, g, depth))
+    return {"ms": round(best[0], 5), "grid": best[1], "depth": best[2],
+            "what": "hbm_probe_gather: A's column ids and values streamed, x gathered at "
+                    "each id, no row walk or y; best of 1-16 workgroups per CU and 1-8 "
+                    "pairs of 16-byte loads (4-32 gathers) in flight per lane, rotating "
+                    "over the copies of A"}
+
+
 def merge_path_legs(copies, x, y, stream, args, nbytes, flops):
     """The reference's named path on the same matrix: per-call
     loops_spmv(merge_path_flat) with HIP events around its main kernel
@@ -186,6 +238,9 @@ def merge_path_legs(copies, x, y, stream, args, nbytes, flops):
         "frac": round(nbytes / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "frac_step": round(nbytes / (step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profile("merge_path_plain")}
+    out["csr_gather_ceiling"] = csr_gather_ceiling(copies, x, stream)
+    out["per_call"]["frac_of_csr_gather_ceiling"] = round(
+        out["csr_gather_ceiling"]["ms"] / kern, 4)
     plan = loops.spmv_plan(copies[0], layout="panels", stream=stream)
     for _ in range(args.warmup):
         plan.execute(x, y, stream)
@@ -206,7 +261,9 @@ def merge_path_legs(copies, x, y, stream, args, nbytes, flops):
         "traffic": traffic_from_profile("panels")}
     plan.destroy()
     log(f"merge_path per call: kernel {out['per_call']['kernel_avg_ms']*1e3:.1f} us "
-        f"(frac {out['per_call']['frac']}); panels plan {out['panels_plan']['kernel_avg_ms']*1e3:.1f} us")
+        f"(frac {out['per_call']['frac']}); CSR-order gather ceiling "
+        f"{out['csr_gather_ceiling']['ms']*1e3:.1f} us; panels plan "
+        f"{out['panels_plan']['kernel_avg_ms']*1e3:.1f} us")
     return out
 
 

@@ -6,6 +6,10 @@
 //
 //   hbm_probe_read(src, bytes, grid, sink, stream)   reads `bytes` once
 //   hbm_probe_copy(dst, src, bytes, grid, stream)    reads and writes them
+//   hbm_probe_gather(ids, vals, n, x, grid, depth, sink, stream)
+//       streams n column ids and values (16-byte non-temporal loads) and
+//       gathers x at every id: a CSR matrix's nonzeros in CSR order without
+//       the row walk or y, the least any CSR-order SpMV reads per nonzero
 //
 // The read kernel folds every word into a per-lane xor and stores it only
 // when it equals a constant no input produces in practice, so the loads
@@ -56,6 +60,40 @@ __global__ __launch_bounds__(kTpb) void hbm_copy_kernel(u32x4* __restrict__ dst,
   for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int kD>  // 16-byte loads of ids and of values in flight per lane: 2 kD
+__global__ __launch_bounds__(kTpb) void hbm_gather_kernel(const i32x4* __restrict__ ids,
+                                                          const f32x4* __restrict__ vals,
+                                                          size_t n4, const int* __restrict__ tid,
+                                                          const float* __restrict__ tval, int ntail,
+                                                          const float* __restrict__ x,
+                                                          unsigned* __restrict__ sink) {
+  const size_t stride = (size_t)gridDim.x * kTpb;
+  size_t i = (size_t)blockIdx.x * kTpb + threadIdx.x;
+  float acc = 0.f;
+  for (; i + (kD - 1) * stride < n4; i += kD * stride) {
+    i32x4 c[kD];
+    f32x4 v[kD];
+#pragma unroll
+    for (int j = 0; j < kD; ++j) {
+      c[j] = __builtin_nontemporal_load(ids + i + j * stride);
+      v[j] = __builtin_nontemporal_load(vals + i + j * stride);
+    }
+#pragma unroll
+    for (int j = 0; j < kD; ++j)
+      acc += v[j].x * x[c[j].x] + v[j].y * x[c[j].y] + v[j].z * x[c[j].z] + v[j].w * x[c[j].w];
+  }
+  for (; i < n4; i += stride) {
+    const i32x4 c = __builtin_nontemporal_load(ids + i);
+    const f32x4 v = __builtin_nontemporal_load(vals + i);
+    acc += v.x * x[c.x] + v.y * x[c.y] + v.z * x[c.z] + v.w * x[c.w];
+  }
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) acc += tval[threadIdx.x] * x[tid[threadIdx.x]];
+  if (__float_as_uint(acc) == 0x9e3779b9u) sink[threadIdx.x] = 1u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -76,6 +114,31 @@ int hbm_probe_copy(void* dst, const void* src, size_t bytes, int grid, hipStream
     return (int)hipErrorInvalidValue;
   hbm_copy_kernel<<<grid, kTpb, 0, stream>>>(static_cast<u32x4*>(dst),
                                              static_cast<const u32x4*>(src), bytes / 16);
+  return (int)hipGetLastError();
+}
+
+// ids, vals: 16-byte aligned, n entries; every id indexes x (the caller's
+// CSR column ids, which it has checked); grid > 0; depth 1, 2, 4 or 8
+// (pairs of 16-byte loads in flight per lane, 4 gathers each); sink as for
+// the read.
+int hbm_probe_gather(const int* ids, const float* vals, size_t n, const float* x, int grid,
+                     int depth, unsigned* sink, hipStream_t stream) {
+  if (!ids || !vals || !x || !sink || grid <= 0 ||
+      reinterpret_cast<uintptr_t>(ids) % 16 != 0 || reinterpret_cast<uintptr_t>(vals) % 16 != 0)
+    return (int)hipErrorInvalidValue;
+  const size_t n4 = n / 4;
+  const i32x4* i4 = reinterpret_cast<const i32x4*>(ids);
+  const f32x4* v4 = reinterpret_cast<const f32x4*>(vals);
+  const int* ti = ids + 4 * n4;
+  const float* tv = vals + 4 * n4;
+  const int nt = (int)(n - 4 * n4);
+  switch (depth) {
+    case 1: hbm_gather_kernel<1><<<grid, kTpb, 0, stream>>>(i4, v4, n4, ti, tv, nt, x, sink); break;
+    case 2: hbm_gather_kernel<2><<<grid, kTpb, 0, stream>>>(i4, v4, n4, ti, tv, nt, x, sink); break;
+    case 4: hbm_gather_kernel<4><<<grid, kTpb, 0, stream>>>(i4, v4, n4, ti, tv, nt, x, sink); break;
+    case 8: hbm_gather_kernel<8><<<grid, kTpb, 0, stream>>>(i4, v4, n4, ti, tv, nt, x, sink); break;
+    default: return (int)hipErrorInvalidValue;
+  }
   return (int)hipGetLastError();
 }
 

@@ -391,6 +391,42 @@ def spmv_bf16_plan(A_h, x_h, reps=20, log=print):
     return out
 
 
+def _rows_gather_ceiling(A, B, passes, reps=5):
+    """Time of the least reading any CSR-order SpMM does for the given
+    column passes ((first column, columns) each): every nonzero's column id
+    and value, and its B row's columns of the pass (hbm_probe_gather_rows),
+    summed over the passes, each the best of 1-16 workgroups per CU and 2-8
+    nonzeros in flight per lane group (ms)."""
+    import ctypes
+    import os
+
+    import torch
+    root = os.path.dirname(os.path.abspath(__file__))
+    lib = ctypes.CDLL(os.path.join(root, "edge_loops_amd", "lib", "libloops_hbm.so"))
+    lib.hbm_probe_gather_rows.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_longlong,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    assert A.indices.dtype == torch.int32 and A.values.dtype == torch.float32
+    assert B.dtype == torch.float32 and B.is_contiguous() and B.shape[0] >= A.cols
+    assert B.data_ptr() % 16 == 0 and B.shape[1] % 4 == 0
+    sink = torch.zeros(256, dtype=torch.int32, device="cuda")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    stream = torch.cuda.current_stream().cuda_stream
+    total = 0.0
+    for c0, k in passes:
+        assert c0 % 4 == 0 and c0 + k <= B.shape[1]
+
+        def run(g, d):
+            rc = lib.hbm_probe_gather_rows(A.indices.data_ptr(), A.values.data_ptr(), A.nnzs,
+                                           B.data_ptr(), B.shape[1], c0, k, g, d,
+                                           sink.data_ptr(), stream)
+            if rc != 0:
+                raise RuntimeError(f"row gather probe failed: hipError {rc}")
+        total += min(_time(lambda: run(cus * per_cu, d), reps)
+                     for per_cu in (1, 2, 4, 8, 16) for d in (2, 4, 8))
+    return total
+
+
 def spmm_c3(reps=5, log=print):
     """SpMM C = A B on the 1M-row power-law matrix (Zipf 2.0, 16M nonzeros)
     with a dense 1M x 64 fp32 B: group_mapped and merge_path, warm, with the
@@ -433,4 +469,17 @@ def spmm_c3(reps=5, log=print):
                       "sampled_rows_beyond_1e-5": bad}
         log(f"spmm c3 {sched}: {ms:.3f} ms, {out[sched]['gflops']:.0f} GFLOP/s, "
             f"{out[sched]['gbs']:.0f} GB/s, bad rows {bad}")
+    # the B-row gathers any CSR-order product of A and B makes: two 32-column
+    # passes (how merge_path runs k = 64) and one 64-column pass
+    g32 = _rows_gather_ceiling(A, B, [(0, 32), (32, 32)], reps)
+    g64 = _rows_gather_ceiling(A, B, [(0, 64)], reps)
+    best = min(g32, g64)
+    out["gather_ceiling"] = {
+        "two_32_column_passes_ms": g32, "one_64_column_pass_ms": g64,
+        "what": "hbm_probe_gather_rows: each nonzero's column id, value and B-row columns "
+                "of the pass (16-byte loads), no row walk or C; best of 1-16 workgroups per "
+                "CU and 2-8 nonzeros in flight per lane group, warm like the timed legs"}
+    for sched in ("group_mapped", "merge_path_flat"):
+        out[sched]["frac_of_gather_ceiling"] = best / out[sched]["ms"]
+    log(f"spmm c3 gather ceiling: {g32:.3f} ms (2 x 32 columns), {g64:.3f} ms (64 columns)")
     return out

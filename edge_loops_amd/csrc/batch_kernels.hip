@@ -58,10 +58,10 @@ enum batch_kind : int { BK_MERGE = 0, BK_ROWS = 1, BK_SPMM = 2, BK_GROUP = 3, BK
 //   SpMV classes 0..2: merge tiles of 4 (and the ROWS / GROUP pieces), 8 and
 //                      16 steps per lane;
 //   SpMM variants:     value type x columns per lane (1, 4) x merge steps
-//                      per range (4, 8, 16), class kSpmvClasses + variant;
+//                      per range (4, 8, 16, 32), class kSpmvClasses + variant;
 //   row-block streams: class kBlkClass.
 constexpr int kSpmvClasses = 3;
-constexpr int kSpmmVariants = 3 * 2 * 3;
+constexpr int kSpmmVariants = 3 * 2 * 4;
 constexpr int kBlkClass = kSpmvClasses + kSpmmVariants;
 constexpr int kClasses = kBlkClass + 1;
 // A plan item of several segments has its streams padded to a multiple of
@@ -73,7 +73,7 @@ constexpr int kClasses = kBlkClass + 1;
 // XCD's L2 holds only its slices of x.  One-segment items are not padded.
 constexpr int kBlkPiecePad = 8;
 __host__ __device__ constexpr int spmm_variant(int vtype, int cpl, int ipt) {
-  return (vtype * 2 + (cpl == 4 ? 1 : 0)) * 3 + (ipt == 4 ? 0 : ipt == 8 ? 1 : 2);
+  return (vtype * 2 + (cpl == 4 ? 1 : 0)) * 4 + (ipt == 4 ? 0 : ipt == 8 ? 1 : ipt == 16 ? 2 : 3);
 }
 __host__ __device__ constexpr int spmv_class(int maxipt) {
   return maxipt == 4 ? 0 : maxipt == 8 ? 1 : 2;
@@ -584,10 +584,10 @@ loops_status_t count_item(const loops_batch_item_t& b, item_counts* c) {
     if (b.schedule != LOOPS_SCHEDULE_MERGE_PATH_FLAT) return LOOPS_ERROR_NOT_SUPPORTED;
     c->kind = BK_SPMM;
     const spmm_pick sp = spmm_shape_for(b);
-    // 0: the per-call default (16 steps for ranges of 16 lanes or more, else
-    // 8), so an item matches loops_spmm with the same parameters bit for bit
-    c->ipt = b.items_per_thread ? b.items_per_thread : (sp.g >= 16 ? 16 : 8);
-    if (c->ipt != 4 && c->ipt != 8 && c->ipt != 16) return LOOPS_ERROR_NOT_SUPPORTED;
+    // 0: the per-call default (spmm_default_ipt), so an item matches
+    // loops_spmm with the same parameters bit for bit
+    c->ipt = b.items_per_thread ? b.items_per_thread : spmm_default_ipt(sp.g);
+    if (!spmm_ipt_ok(c->ipt, sp.g)) return LOOPS_ERROR_NOT_SUPPORTED;
     c->g = sp.g;
     c->cpl = sp.cpl;
     c->ti = (int)spmm_tile_items(c->g, c->ipt);
@@ -849,17 +849,19 @@ void spmm_kernels_of(spmm_kernel_fn* k) {
   k[0] = batch_spmm_kernel<4, 1, V>;
   k[1] = batch_spmm_kernel<8, 1, V>;
   k[2] = batch_spmm_kernel<16, 1, V>;
-  k[3] = batch_spmm_kernel<4, 4, V>;
-  k[4] = batch_spmm_kernel<8, 4, V>;
-  k[5] = batch_spmm_kernel<16, 4, V>;
+  k[3] = batch_spmm_kernel<32, 1, V>;
+  k[4] = batch_spmm_kernel<4, 4, V>;
+  k[5] = batch_spmm_kernel<8, 4, V>;
+  k[6] = batch_spmm_kernel<16, 4, V>;
+  k[7] = batch_spmm_kernel<32, 4, V>;
 }
 
 struct spmm_kernel_table {
   spmm_kernel_fn fn[kSpmmVariants];
   spmm_kernel_table() {
-    spmm_kernels_of<float>(fn + 6 * LOOPS_VALUE_F32);
-    spmm_kernels_of<bf16_t>(fn + 6 * LOOPS_VALUE_BF16);
-    spmm_kernels_of<f16_t>(fn + 6 * LOOPS_VALUE_F16);
+    spmm_kernels_of<float>(fn + 8 * LOOPS_VALUE_F32);
+    spmm_kernels_of<bf16_t>(fn + 8 * LOOPS_VALUE_BF16);
+    spmm_kernels_of<f16_t>(fn + 8 * LOOPS_VALUE_F16);
   }
 };
 const spmm_kernel_table g_spmm_kernels;

@@ -683,7 +683,8 @@ loops_status_t loops_spmm(loops_handle_t handle, loops_schedule_t schedule,
     case LOOPS_SCHEDULE_MERGE_PATH_FLAT: {
       int tpb = p->threads_per_block ? p->threads_per_block : 256;
       int ipw = p->items_per_thread ? p->items_per_thread : 0;
-      if (!mp_spmm_supported(tpb, ipw)) return LOOPS_ERROR_NOT_SUPPORTED;
+      if (!mp_spmm_supported(tpb, ipw) || !mp_spmm_fits((int)k, B, ldb, ipw))
+        return LOOPS_ERROR_NOT_SUPPORTED;
       size_t need = mp_spmm_ws_need(a, (int)k, ipw);
       void* ws = nullptr;
       st = get_workspace(handle, need, workspace, workspace_bytes, &ws);

@@ -10,6 +10,11 @@
 //       streams n column ids and values (16-byte non-temporal loads) and
 //       gathers x at every id: a CSR matrix's nonzeros in CSR order without
 //       the row walk or y, the least any CSR-order SpMV reads per nonzero
+//   hbm_probe_gather_rows(ids, vals, n, B, ldb, c0, k, grid, depth, sink,
+//                         stream)
+//       for each of n nonzeros reads its column id and value and the k
+//       floats B[id][c0 .. c0 + k) (16-byte loads, k / 4 lanes per row):
+//       the least any CSR-order SpMM of those k columns reads per nonzero
 //
 // The read kernel folds every word into a per-lane xor and stores it only
 // when it equals a constant no input produces in practice, so the loads
@@ -94,6 +99,35 @@ __global__ __launch_bounds__(kTpb) void hbm_gather_kernel(const i32x4* __restric
   if (__float_as_uint(acc) == 0x9e3779b9u) sink[threadIdx.x] = 1u;
 }
 
+// G lanes per nonzero (4 columns each), D nonzeros in flight per lane group.
+template <int G, int D>
+__global__ __launch_bounds__(kTpb) void hbm_gather_rows_kernel(
+    const int* __restrict__ ids, const float* __restrict__ vals, size_t n,
+    const float* __restrict__ B, long long ldb, int c0, unsigned* __restrict__ sink) {
+  const size_t groups = (size_t)gridDim.x * (kTpb / G);
+  const int sub = threadIdx.x % G;
+  size_t nz = (size_t)blockIdx.x * (kTpb / G) + threadIdx.x / G;
+  const float* __restrict__ Bc = B + c0 + 4 * sub;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (; nz + (D - 1) * groups < n; nz += D * groups) {
+    int c[D];
+    float v[D];
+    f32x4 b[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      c[j] = ids[nz + j * groups];
+      v[j] = vals[nz + j * groups];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) b[j] = *reinterpret_cast<const f32x4*>(Bc + (long long)c[j] * ldb);
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc += v[j] * b[j];
+  }
+  for (; nz < n; nz += groups)
+    acc += vals[nz] * *reinterpret_cast<const f32x4*>(Bc + (long long)ids[nz] * ldb);
+  if (__float_as_uint(acc.x + acc.y + acc.z + acc.w) == 0x9e3779b9u) sink[threadIdx.x] = 1u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -140,6 +174,28 @@ int hbm_probe_gather(const int* ids, const float* vals, size_t n, const float* x
     default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();
+}
+
+// ids, vals: n entries, every id a row of B; B, ldb and c0 such that every
+// B + id * ldb + c0 is 16-byte aligned; k 4, 8, 16, 32 or 64 columns;
+// depth 2, 4 or 8 nonzeros in flight per lane group; grid > 0.
+int hbm_probe_gather_rows(const int* ids, const float* vals, size_t n, const float* B,
+                          long long ldb, int c0, int k, int grid, int depth, unsigned* sink,
+                          hipStream_t stream) {
+  if (!ids || !vals || !B || !sink || grid <= 0 || c0 < 0 || ldb % 4 != 0 || c0 % 4 != 0 ||
+      c0 + k > ldb || reinterpret_cast<uintptr_t>(B) % 16 != 0)
+    return (int)hipErrorInvalidValue;
+#define LOOPS_ROWS_PROBE(G, D)                                                          \
+  if (k == 4 * G && depth == D) {                                                       \
+    hbm_gather_rows_kernel<G, D><<<grid, kTpb, 0, stream>>>(ids, vals, n, B, ldb, c0, sink); \
+    return (int)hipGetLastError();                                                      \
+  }
+#define LOOPS_ROWS_PROBE_G(G) LOOPS_ROWS_PROBE(G, 2) LOOPS_ROWS_PROBE(G, 4) LOOPS_ROWS_PROBE(G, 8)
+  LOOPS_ROWS_PROBE_G(1) LOOPS_ROWS_PROBE_G(2) LOOPS_ROWS_PROBE_G(4) LOOPS_ROWS_PROBE_G(8)
+  LOOPS_ROWS_PROBE_G(16)
+#undef LOOPS_ROWS_PROBE_G
+#undef LOOPS_ROWS_PROBE
+  return (int)hipErrorInvalidValue;
 }
 
 }  // extern "C"

@@ -206,13 +206,33 @@ hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ld
                           float* C, long long ldc, int tpb, hipEvent_t ev0,
                           hipEvent_t ev1, hipStream_t s);
 bool mp_spmm_supported(int tpb, int ipt);
+// Whether every column chunk of a k-column product with this B takes ipt
+// steps per range (spmm_ipt_ok for the chunk's lanes per range).
+bool mp_spmm_fits(int k, const float* B, long long ldb, int ipt);
 // Merge-path SpMM with more than kSpmmChunk columns runs as launches of
 // kSpmmChunk columns (the last one the rest), one after the other on the
-// stream: ranges of 8 lanes x 4 columns with 8 steps beat the wider shapes
-// (1M-row, 16M-nonzero Zipf(2.0) matrix, fp32: k = 64 1.53 -> 1.27 ms,
-// k = 48 1.56 -> 1.10, k = 128 3.84 -> 2.53).  Batches split such items the
-// same way, so an item's result stays the one loops_spmm gives.
-constexpr int kSpmmChunk = 32;
+// stream.  With the range's B rows loaded 8 steps at a time, ranges of 16
+// lanes x 4 columns with 32 steps beat two launches of 8-lane ranges
+// (1M-row, 16M-nonzero Zipf(2.0) matrix, fp32, k = 64: 0.70 ms against
+// 2 x 0.375; 400k-row, 2M-nonzero matrix: 0.131 against 2 x 0.082).
+// Batches split such items the same way, so an item's result stays the one
+// loops_spmm gives.
+constexpr int kSpmmChunk = 64;
+// Merge steps per range of a merge-path SpMM tile when the caller passes 0:
+// 32 for ranges of 16 lanes or more, else 16.  The range loads its B rows 8
+// steps at a time, so longer ranges cost no registers and spread the tile's
+// fixed work (end-point searches, staging, the scan across ranges) over
+// more nonzeros.  1M-row, 16M-nonzero Zipf(2.0) matrix, microseconds at 4 /
+// 8 / 16 / 32 steps: k = 1 178 / 156 / 145 / -, k = 4 274 / 247 / 242 / -,
+// k = 16 343 / 325 / 310 / -, k = 32 555 / 433 / 378 / 375, k = 64 1258 /
+// 938 / 747 / 700; 400k-row, 2M-nonzero matrix: k = 1 39 / 37 / 32 / -,
+// k = 16 71 / 69 / 71 / -, k = 64 175 / 145 / 131 / 133.
+inline int spmm_default_ipt(int g) { return g >= 16 ? 32 : 16; }
+// Steps per range a merge-path SpMM tile takes: 4, 8, 16, or 32 for ranges
+// of 8 lanes or more (the tile's LDS holds 1,024 steps at 32).
+inline bool spmm_ipt_ok(int ipt, int g) {
+  return ipt == 4 || ipt == 8 || ipt == 16 || (ipt == 32 && g >= 8);
+}
 // SpMM lanes take 4 consecutive columns (16-byte B loads) when B's rows are
 // 16-byte aligned (B and ldb) and k is at least this, else 1 column.  C is
 // written with 16-byte stores where its address allows, else column by
@@ -223,7 +243,7 @@ constexpr int kSpmmChunk = 32;
 // lanes).  Ranges hold the fewest lanes (at least 2) that span k columns.
 constexpr int kSpmmVec4MinK = 8;
 // Width of the chunk starting at column j of a k-column product (k >
-// kSpmmChunk): kSpmmChunk, except that 33 columns left go as 28 + 5, never
+// kSpmmChunk): kSpmmChunk, except that 65 columns left go as 60 + 5, never
 // leaving a one-column chunk (a one-column batch item is an SpMV, which
 // reads x and y as vectors) and keeping chunk starts 16-byte aligned.
 inline int spmm_chunk_width(long long k, long long j) {

@@ -362,7 +362,7 @@ hipError_t mp_spmm_v(const csr_args& A, int k, const float* B, long long ldb,
 #define X(I_, C_)                 \
   if (ipt == I_ && sh.cpl == C_) \
     return mp_spmm_t<I_, C_, V>(A, k, B, ldb, C, ldc, sh.g, ws, s, ev1);
-  X(4, 1) X(8, 1) X(16, 1) X(4, 4) X(8, 4) X(16, 4)
+  X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(4, 4) X(8, 4) X(16, 4) X(32, 4)
 #undef X
   return hipErrorInvalidValue;
 }
@@ -379,12 +379,7 @@ spmm_shape tile_shape(int k, bool vec4) {
   return sh;
 }
 
-// Merge steps per range when the caller passes 0: 16 for ranges of 16 lanes
-// or more (16 ranges or fewer per tile), else 8, so a tile is not a handful
-// of steps for wide k.  Measured on the 1M-row, 16M-nonzero Zipf(2.0)
-// matrix: k = 64 (16-lane ranges) 1.52 ms at 16 steps, 1.65 at 8, 2.29 at
-// 4; k = 16 (4-lane ranges) 0.386 ms at 8, 0.394 at 4, 0.440 at 16.
-inline int default_ipt(const spmm_shape& sh) { return sh.g >= 16 ? 16 : 8; }
+inline int default_ipt(const spmm_shape& sh) { return spmm_default_ipt(sh.g); }
 
 }  // namespace
 
@@ -421,7 +416,16 @@ hipError_t launch_gm_spmm(const csr_args& A, int k, const float* B, long long ld
 }
 
 bool mp_spmm_supported(int tpb, int ipt) {
-  return tpb == 256 && (ipt == 0 || ipt == 4 || ipt == 8 || ipt == 16);
+  return tpb == 256 && (ipt == 0 || ipt == 4 || ipt == 8 || ipt == 16 || ipt == 32);
+}
+
+bool mp_spmm_fits(int k, const float* B, long long ldb, int ipt) {
+  if (ipt == 0) return true;
+  for (int j = 0, w = 0; j < k; j += w) {
+    w = k > kSpmmChunk ? spmm_chunk_width(k, j) : k;
+    if (!spmm_ipt_ok(ipt, tile_shape(w, vec4_ok(B + j, ldb)).g)) return false;
+  }
+  return true;
 }
 
 // Tiles of the launch: the most over both shapes (4 columns per lane when B

@@ -96,8 +96,9 @@ inline spmm_ws carve_spmm_ws(void* ws, long long units, int k) {
 //     wavefronts with the 64-way cooperative search.
 //  2. Row ends, column ids and values of the tile go to LDS (coalesced).
 //  3. Each range finds its start and end in LDS, then issues the B-row loads
-//     of all its nonzeros at once (one round trip) and walks its steps,
-//     storing each row it closes except the first, which it holds back.
+//     of up to 8 nonzeros at once (one round trip per 8 steps) and walks
+//     those steps, storing each row it closes except the first, which it
+//     holds back.
 //  4. A segmented scan over the ranges (per column) carries partial sums
 //     between ranges; the held-back rows are completed with it.
 //  5. The row entering the tile open goes to head[tile], the row leaving it
@@ -110,7 +111,9 @@ constexpr int kSpmmMinG = 2;
 
 template <int IPT, int CPL>
 struct spmm_smem {
-  static constexpr int TI_MAX = (kSpmmTpb / kSpmmMinG) * IPT;
+  // tiles of up to 16 steps per range for ranges of kSpmmMinG lanes or
+  // more; 32 steps for ranges of 8 lanes or more (spmm_ipt_ok)
+  static constexpr int TI_MAX = (kSpmmTpb / (IPT <= 16 ? kSpmmMinG : 8)) * IPT;
   static constexpr int NW = kSpmmTpb / 64;
   int rowend[TI_MAX + 1];
   int col[TI_MAX];
@@ -223,35 +226,16 @@ __device__ __forceinline__ void spmm_tile(
 
   for (int cb = 0; cb < k; cb += G * CPL) {
     const int c = cb + g * CPL;
-    // 3b + 3c: the B rows of this range's nonzeros, all loads in flight
+    // 3b + 3c: the B rows of this range's nonzeros, 8 loads in flight
     // together, then the walk over its steps.
     float acc[CPL], first_acc[CPL];
     int x = xl, first_row = -1;
     auto load_walk = [&](auto whole) {
       constexpr bool kWhole = decltype(whole)::value;
-      cols_t<CPL> bj[IPT];
-      float vj[IPT];
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        if constexpr (kWhole) {
-          // every lane's columns lie inside B's rows and the tile holds
-          // nonzeros: a load per step whatever cnt is (steps past cnt
-          // read the tile's first B row and are not summed), so no load
-          // sits under a branch and none is waited for before the walk
-          const int jj = j < cnt ? yl + j : 0;
-          vj[j] = sm.val[jj];
-          const float* p = B + (long long)sm.col[jj] * ldb + c;
-          if constexpr (CPL == 4) {
-            const f32x4 q = *reinterpret_cast<const f32x4*>(p);
-            bj[j].v[0] = q.x; bj[j].v[1] = q.y; bj[j].v[2] = q.z; bj[j].v[3] = q.w;
-          } else {
-            bj[j].v[0] = *p;
-          }
-        } else if (j < cnt) {
-          vj[j] = sm.val[yl + j];
-          bj[j] = load_b<CPL>(B, ldb, sm.col[yl + j], c, k);
-        }
-      }
+      // B rows are loaded H steps at a time and walked before the next H
+      // are loaded: the same sums in the same order as loading all IPT at
+      // once, in a fixed register budget
+      constexpr int H = IPT < 8 ? IPT : 8;
 #pragma unroll
       for (int q = 0; q < CPL; ++q) acc[q] = first_acc[q] = 0.f;
       int row_end = sm.rowend[x];
@@ -270,12 +254,38 @@ __device__ __forceinline__ void spmm_tile(
       };
       int yy = yl;
 #pragma unroll
-      for (int j = 0; j < IPT; ++j) {
-        if (j < cnt) {
-          while (row_end <= yy) close_row();
+      for (int h = 0; h < IPT; h += H) {
+        cols_t<CPL> bj[H];
+        float vj[H];
 #pragma unroll
-          for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
-          ++yy;
+        for (int j = 0; j < H; ++j) {
+          if constexpr (kWhole) {
+            // every lane's columns lie inside B's rows and the tile holds
+            // nonzeros: a load per step whatever cnt is (steps past cnt
+            // read the tile's first B row and are not summed), so no load
+            // sits under a branch and none is waited for before the walk
+            const int jj = h + j < cnt ? yl + h + j : 0;
+            vj[j] = sm.val[jj];
+            const float* p = B + (long long)sm.col[jj] * ldb + c;
+            if constexpr (CPL == 4) {
+              const f32x4 q = *reinterpret_cast<const f32x4*>(p);
+              bj[j].v[0] = q.x; bj[j].v[1] = q.y; bj[j].v[2] = q.z; bj[j].v[3] = q.w;
+            } else {
+              bj[j].v[0] = *p;
+            }
+          } else if (h + j < cnt) {
+            vj[j] = sm.val[yl + h + j];
+            bj[j] = load_b<CPL>(B, ldb, sm.col[yl + h + j], c, k);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+          if (h + j < cnt) {
+            while (row_end <= yy) close_row();
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) acc[q] += vj[j] * bj[j].v[q];
+            ++yy;
+          }
         }
       }
       while (x < xl2) close_row();

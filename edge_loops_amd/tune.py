@@ -11,14 +11,16 @@ group-mapped).  Here the candidates are the batch kernel's piece kinds:
           (spmv_plan(..., layout="blocks", for_batches=True)), measured for
           buckets whose matrices hold at least BLOCKS_MIN_NNZ nonzeros on
           average;
-  k >  1: merge_path_flat SpMM tiles of 4, 8 or 16 merge steps per range.
+  k >  1: merge_path_flat SpMM tiles of 4, 8 or 16 merge steps per range,
+          or 0: the library's default (16 steps, 32 for ranges of 16 lanes
+          or more).
 
 For each bucket the tuning matrices that fall in it are run as one batch per
 candidate (items repeated until the batch holds enough work to fill the
 GPU), captured in a graph and replayed; the fastest candidate wins.  The
 table is plain JSON: bucket key -> {"schedule", "tile", "us": {candidate:
 microseconds}}.  Lookup falls back to the nearest tuned bucket, then to
-merge_path_flat with 8 steps per lane (SpMM: 8 steps per range); a bucket's
+merge_path_flat with 8 steps per lane (SpMM: the library's default); a bucket's
 measured winner is used only when it beats that default by PICK_MARGIN.
 A "blocks" pick asks the caller for a plan per matrix (built once; a copy
 of A in the row-block layout); callers that take no plans ask for picks
@@ -34,9 +36,9 @@ import os
 import numpy as np
 
 SPMV_MERGE = [4, 8, 16]
-SPMM_IPT = [4, 8, 16]
+SPMM_IPT = [0, 4, 8, 16]
 DEFAULT_SPMV = ("merge_path_flat", 8)
-DEFAULT_SPMM = ("merge_path_flat", 8)
+DEFAULT_SPMM = ("merge_path_flat", 0)
 # A measured pick replaces the default only when it is at least this much
 # faster: smaller differences are within run-to-run noise, and in a mixed
 # batch every extra tile class costs a persistent kernel of its own.

@@ -337,7 +337,8 @@ loops_status_t loops_spmv_plan_destroy(loops_spmv_plan_t plan);
  *           length), reduced in a fixed order;
  *   k >  1, LOOPS_SCHEDULE_MERGE_PATH_FLAT: merge-path units of lane
  *           groups spanning the k columns, items_per_thread steps per group
- *           (4, 8 or 16; 0 = the loops_spmm default);
+ *           (4, 8, 16, or 32 for groups of 8 lanes or more; 0 = the
+ *           loops_spmm default);
  *   k == 1, plan != NULL: the row-block streams of a BLOCKS plan of A
  *           (loops_spmv_plan_create), bit for bit what
  *           loops_spmv_plan_execute gives; schedule, items_per_thread and
@@ -506,14 +507,16 @@ loops_status_t loops_server_replay(loops_server_t server,
  * (container/matrix.cuh:27-35) with leading dimensions ldb, ldc >= k.
  * Schedules: THREAD_MAPPED (algorithms/spmm/thread_mapped.cuh:28-53),
  * GROUP_MAPPED and MERGE_PATH_FLAT (new in this library; the reference has
- * only thread_mapped SpMM).  MERGE_PATH_FLAT: more than 32 columns run as
- * launches of 32 columns each, in order on the stream (33 columns left go
- * as 28 + 5).  Lanes take 4 columns (16-byte B loads) from 8 columns up
- * when B's rows are 16-byte aligned (B and ldb), else 1; C's alignment does
- * not matter.  params->items_per_thread merge steps per lane range in each
- * launch (4, 8 or 16; 0 = 16 when a range spans 16 lanes or more, i.e.
- * above 8 columns without 16-byte aligned B rows, else 8); the workspace
- * size covers the widest launch. */
+ * only thread_mapped SpMM).  MERGE_PATH_FLAT: more than 64 columns run as
+ * launches of 64 columns each, in order on the stream (65 columns left go
+ * as 60 + 5).  Lanes take 4 columns (16-byte B loads) from 8 columns up
+ * when B's rows are 16-byte aligned (B and ldb), else 1; a range is the
+ * fewest lanes (at least 2, a power of two) that span a launch's columns;
+ * C's alignment does not matter.  params->items_per_thread merge steps per
+ * lane range in each launch: 4, 8, 16, or 32 when every launch's ranges
+ * hold 8 lanes or more (else LOOPS_ERROR_NOT_SUPPORTED); 0 = 32 for ranges
+ * of 16 lanes or more, else 16.  The workspace size covers the widest
+ * launch. */
 loops_status_t loops_spmm_workspace_size(loops_handle_t handle,
                                          loops_schedule_t schedule,
                                          const loops_csr_view_t* A, int64_t k,

@@ -34,34 +34,41 @@ def test_layout_counts():
     st, info = query([item(1000, 9000, schedule="group_mapped", ipt=16)])
     assert st == _capi.SUCCESS and info.work_items == 63
     # k = 8 with 16-byte B rows: 4 columns per lane, ranges of 2 lanes,
-    # 128 ranges x 8 steps = 1024 steps per tile
+    # 128 ranges x 16 steps (the default) = 2048 steps per tile
     st, info = query([item(1000, 9000, k=8)])
     assert st == _capi.SUCCESS
-    assert (info.spmm_units, info.spmm_work_items, info.work_items) == (10, 10, 10)
-    assert info.coordinates == 11
-    # k = 7: 1 column per lane, ranges of 8 lanes, 32 x 8 = 256 steps per tile
+    assert (info.spmm_units, info.spmm_work_items, info.work_items) == (5, 5, 5)
+    assert info.coordinates == 6
+    # k = 7: 1 column per lane, ranges of 8 lanes, 32 x 16 = 512 steps per tile
     st, info = query([item(1000, 9000, k=7, ldx=8)])
-    assert st == _capi.SUCCESS and info.spmm_units == 40
-    # k = 32 with 16-byte rows: 4 columns per lane, 8 lanes -> 256-step tiles at ipt 8
+    assert st == _capi.SUCCESS and info.spmm_units == 20
+    # k = 32 with 16-byte rows: 4 columns per lane, 8 lanes -> 512-step tiles at ipt 16
     st, info = query([item(1000, 9000, k=32, ipt=16)])
     assert st == _capi.SUCCESS and info.spmm_units == 20
-    # k = 72: chunks of 32, 32 and 8 columns (as loops_spmm launches them):
-    # 40 + 40 units of 256 steps (8 lanes x 4 columns, 8 steps) and 10 of
-    # 1024 steps for the 8 columns (2 lanes x 4 columns, 8 steps)
+    # k = 32 at 32 steps per range (ranges of 8 lanes): 1024-step tiles
+    st, info = query([item(1000, 9000, k=32, ipt=32)])
+    assert st == _capi.SUCCESS and info.spmm_units == 10
+    # k = 72: chunks of 64 and 8 columns (as loops_spmm launches them): 20
+    # units of 512 steps (16 lanes x 4 columns, 32 steps) and 5 of 2048
+    # steps for the 8 columns (2 lanes x 4 columns, 16 steps)
     st, info = query([item(1000, 9000, k=72, ldx=72, ldy=72)])
     assert st == _capi.SUCCESS
-    assert (info.spmm_units, info.coordinates) == (90, 93)
-    # k = 33: 28 + 5 columns, never a one-column chunk (that would be an
-    # SpMV item): 40 units of 256 steps, then 5 columns (8 lanes x 1 column)
+    assert (info.spmm_units, info.coordinates) == (25, 27)
+    # k = 33: one chunk, ranges of 16 lanes x 4 columns, 32 steps
     st, info = query([item(1000, 9000, k=33, ldx=36, ldy=36)])
-    assert st == _capi.SUCCESS and info.spmm_units == 80 and info.spmv_tiles == 0
+    assert st == _capi.SUCCESS and info.spmm_units == 20 and info.spmv_tiles == 0
+    # k = 65: 60 + 5 columns, never a one-column chunk (that would be an
+    # SpMV item): 20 units of 512 steps, then 5 columns (8 lanes x 1 column,
+    # 16 steps: 512 steps per tile)
+    st, info = query([item(1000, 9000, k=65, ldx=68, ldy=68)])
+    assert st == _capi.SUCCESS and info.spmm_units == 40 and info.spmv_tiles == 0
     # a wide item is validated whole: ldx < k is rejected even though each
     # chunk would fit
     assert query([item(1000, 9000, k=72, ldx=40)])[0] == _capi.ERROR_INVALID_VALUE
     st, info = query([item(1000, 9000, ipt=4), item(1000, 9000, k=8), item(50, 0, ipt=16)])
     assert st == _capi.SUCCESS
-    assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 11 + 2
-    assert info.work_items == 21 and info.max_items_per_thread == 16
+    assert info.spmv_tiles == 10 + 1 and info.coordinates == 11 + 6 + 2
+    assert info.work_items == 16 and info.max_items_per_thread == 16
     assert info.table_bytes > 0 and info.workspace_bytes > 0
 
 
@@ -71,7 +78,9 @@ def test_layout_rejections():
     assert query([item(1000, 9000, schedule="group_mapped", ipt=3)])[0] == NS
     assert query([item(1000, 9000, schedule="work_oriented")])[0] == NS
     assert query([item(1000, 9000, ipt=5)])[0] == NS
-    assert query([item(1000, 9000, k=4, ipt=32)])[0] == NS         # SpMM tile: 4, 8, 16
+    assert query([item(1000, 9000, k=4, ipt=32)])[0] == NS         # 32 steps: 8+ lanes
+    assert query([item(1000, 9000, k=16, ipt=32)])[0] == NS        # 4 lanes x 4 columns
+    assert query([item(1000, 9000, k=16, ipt=64)])[0] == NS
     assert query([item(1000, 9000, k=0)])[0] == IV
     assert query([item(1000, 9000, k=8, ldx=4)])[0] == IV          # ldx < k
     assert query([item(-1, 0)])[0] == IV
@@ -143,7 +152,7 @@ def test_table_size_grows_with_items_not_with_their_sizes():
 
 
 def test_spmm_variant_mask():
-    # Variant bit = (value type * 2 + (4 columns per lane)) * 3 + (steps 4/8/16 -> 0/1/2).
+    # Variant bit = (value type * 2 + (4 columns per lane)) * 4 + (steps 4/8/16/32 -> 0..3).
     # With the fake addresses B rows are 16-byte aligned, so k >= 8 with ldx a
     # multiple of 4 takes 4 columns per lane.
     def mask(items):
@@ -152,11 +161,14 @@ def test_spmm_variant_mask():
         return info.spmm_variant_mask
     assert mask([item(1000, 9000)]) == 0                                   # SpMV only
     assert mask([item(1000, 9000, k=7, ipt=8)]) == 1 << 1                  # f32, 1 col, 8
-    assert mask([item(1000, 9000, k=8, ipt=8)]) == 1 << 4                  # f32, 4 cols, 8
+    assert mask([item(1000, 9000, k=8, ipt=8)]) == 1 << 5                  # f32, 4 cols, 8
     assert mask([item(1000, 9000, k=12, ipt=8, ldx=14)]) == 1 << 1         # ldx % 4: 1 col
-    assert mask([item(1000, 9000, k=16, ipt=4)]) == 1 << 3                 # f32, 4 cols, 4
-    assert mask([item(1000, 9000, k=16, ipt=16, vt=_capi.VALUE_BF16)]) == 1 << 11
-    assert mask([item(1000, 9000, k=5, ipt=16, vt=_capi.VALUE_F16)]) == 1 << 14
+    assert mask([item(1000, 9000, k=16, ipt=4)]) == 1 << 4                 # f32, 4 cols, 4
+    assert mask([item(1000, 9000, k=16, ipt=16, vt=_capi.VALUE_BF16)]) == 1 << 14
+    assert mask([item(1000, 9000, k=5, ipt=16, vt=_capi.VALUE_F16)]) == 1 << 18
+    assert mask([item(1000, 9000, k=32, ipt=32)]) == 1 << 7                # f32, 4 cols, 32
+    assert mask([item(1000, 9000, k=64)]) == 1 << 7                        # default: 32
+    assert mask([item(1000, 9000, k=16)]) == 1 << 6                        # default: 16
     assert mask([item(1000, 9000, k=7, ipt=8), item(1000, 9000, k=7, ipt=8),
                  item(0, 0, k=7, ipt=4)]) == 1 << 1                        # no pieces: no bit
     assert mask([item(10, 0, k=7, ipt=4)]) == 1 << 0                       # empty rows are work
@@ -172,7 +184,7 @@ def test_pick_margin_keeps_the_default_for_small_gains():
     assert t.pick(1000, 12000, 400) == ("merge_path_flat", 16)    # 10 % > margin
     kkey = tune.features(1000, 12000, 400, 16)
     t.data["spmm"][kkey] = {"schedule": "merge_path_flat", "tile": "4",
-                            "us": {"merge_path_flat:4": 99.0, "merge_path_flat:8": 100.0}}
+                            "us": {"merge_path_flat:4": 99.0, "merge_path_flat:0": 100.0}}
     assert t.pick(1000, 12000, 400, 16) == tune.DEFAULT_SPMM
 
 

@@ -92,9 +92,27 @@ def test_spmm_merge_path_tile_sizes(ipt, k):
     assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30)
 
 
+@pytest.mark.parametrize("k", [7, 33, 64, 300])
+def test_spmm_merge_path_32_steps(k):
+    """32 steps per range, for launches whose ranges hold 8 lanes or more
+    (7 columns: 8 lanes x 1 column; 33 columns, B rows of 33 floats: 64
+    lanes x 1 column; 64: 16 lanes x 4 columns; 300: four 64-column
+    launches and one of 44, 16 lanes x 4 columns each)."""
+    off, idx, val, cols = MATS["powerlaw_20k"]
+    C, Bh, _ = _run("merge_path", "powerlaw_20k", k, items_per_thread=32)
+    C64, A64 = oracle.spmm_f64(off, idx, val, Bh)
+    assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30)
+
+
 def test_spmm_merge_path_rejects_other_tiles():
     with pytest.raises(loops.LoopsError):
         _run("merge_path", "powerlaw_20k", 8, items_per_thread=256)
+    # 32 steps need ranges of 8 lanes: 2 columns (2 lanes x 1) and 16
+    # columns (4 lanes x 4) are refused before any launch
+    for k in (2, 16):
+        with pytest.raises(loops.LoopsError) as e:
+            _run("merge_path", "powerlaw_20k", k, items_per_thread=32)
+        assert e.value.status == loops._capi.ERROR_NOT_SUPPORTED
 
 
 def test_spmm_deterministic():

@@ -411,16 +411,61 @@ __device__ __forceinline__ void spmm_fixup_short(int k, long long t, long long t
 }
 
 // spmm_fixup_row: the calling wavefront splits its 64 lanes into kp column
-// lanes (kp = k rounded up to a power of two, at most 64) x 64 / kp tile
-// partitions; partition p sums carries t + p, t + p + 64 / kp, ... < t_end
-// for its column, an xor butterfly adds the partitions, and head[t_end] is
-// added last.  The order is fixed by k alone.  A partition loads eight
-// carries ahead of its additions (a row of a 200,000 nonzeros spans ~800
-// tiles), then adds them in order.
+// lanes x 64 / kp tile partitions; partition p sums carries t + p,
+// t + p + 64 / kp, ... < t_end for its columns, an xor butterfly adds the
+// partitions, and head[t_end] is added last.  When k is a multiple of 4 a
+// lane takes 4 columns (kp = k / 4 rounded up to a power of two, at most
+// 64), so a 64-column row has 4 partitions instead of 1; else 1 column (kp
+// = k rounded up, at most 64).  The order is fixed by k alone (16-byte
+// loads where the slots allow, the same sums either way).  A partition
+// loads eight carries ahead of its additions (a row of 200,000 nonzeros
+// spans ~400 tiles of 512 steps), then adds them in order.
 __device__ __forceinline__ void spmm_fixup_row(int k, long long t, long long t_end, int r,
                                                const spmm_ws& ws, float* __restrict__ C,
                                                long long ldc) {
   const int lane = threadIdx.x & 63;
+  if ((k & 3) == 0) {
+    const int kq = k >> 2;  // groups of 4 columns
+    int kp = 1;
+    while (kp < kq && kp < 64) kp <<= 1;
+    const int parts = 64 / kp;
+    const int cl = lane & (kp - 1), part = lane / kp;
+    const bool vec = ((reinterpret_cast<uintptr_t>(ws.carry) |
+                       reinterpret_cast<uintptr_t>(ws.head)) & 15) == 0;
+    auto ld4 = [&](const float* p) {
+      if (vec) return *reinterpret_cast<const f32x4*>(p);
+      f32x4 q;
+      q.x = p[0]; q.y = p[1]; q.z = p[2]; q.w = p[3];
+      return q;
+    };
+    for (int q0 = 0; q0 < kq; q0 += kp) {
+      const int c = 4 * (q0 + cl);
+      f32x4 s = {0.f, 0.f, 0.f, 0.f};
+      if (c < k) {
+        long long v = t + part;
+        for (; v + 7LL * parts < t_end; v += 8LL * parts) {
+          f32x4 g[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) g[u] = ld4(ws.carry + (v + (long long)u * parts) * k + c);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s += g[u];
+        }
+        for (; v < t_end; v += parts) s += ld4(ws.carry + v * k + c);
+      }
+      for (int m = kp; m < 64; m <<= 1) {
+        s.x += __shfl_xor(s.x, m, 64);
+        s.y += __shfl_xor(s.y, m, 64);
+        s.z += __shfl_xor(s.z, m, 64);
+        s.w += __shfl_xor(s.w, m, 64);
+      }
+      if (part == 0 && c < k) {
+        const f32x4 h = ld4(ws.head + t_end * k + c);
+        const float out[4] = {s.x + h.x, s.y + h.y, s.z + h.z, s.w + h.w};
+        store_c<4>(C + (long long)r * ldc + c, c, k, out);
+      }
+    }
+    return;
+  }
   int kp = 1;
   while (kp < k && kp < 64) kp <<= 1;
   const int parts = 64 / kp;

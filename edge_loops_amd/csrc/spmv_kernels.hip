@@ -242,18 +242,44 @@ __global__ __launch_bounds__(TPB) void spmv_group_mapped_kernel(
       trip = storage.tile_aggregates[g] > trip ? storage.tile_aggregates[g] : trip;
   }
   constexpr int kSentinel = TPT > 64 ? TPB : TPT;
-  for (int base = 0; base < trip; base += TPT) {
-    int v_atom = base + rank;
-    int key = kSentinel + (TPT > 64 ? (int)p.meta_group_rank() : 0);
-    float prod = 0.f;
-    if (v_atom < aggregate) {
-      int v_tile = config.tile_accessor(v_atom, p);
-      int row = config.tile_id(v_tile, p);
-      int nz = config.atom_id(v_atom, row, v_tile, p);
-      prod = value_traits<V>::load1(values + nz) * x[ld_idx(indices + nz)];
-      key = TPT > 64 ? gbase + v_tile : v_tile;
+  // kGmAhead chunks at a time: their atoms are mapped and all their loads
+  // issued (branch-free: a lane past its group's nonzeros loads nonzero 0
+  // and drops the product) before the first is reduced, so a group waits on
+  // memory once per kGmAhead chunks; the chunks are then reduced and added
+  // in order, as one at a time would be.
+  constexpr int kGmAhead = 8;
+  for (int base0 = 0; base0 < trip; base0 += kGmAhead * TPT) {
+    int keys[kGmAhead], nzs[kGmAhead];
+    float prods[kGmAhead];
+#pragma unroll
+    for (int j = 0; j < kGmAhead; ++j) {
+      int v_atom = base0 + j * TPT + rank;  // the accessors take references
+      keys[j] = kSentinel + (TPT > 64 ? (int)p.meta_group_rank() : 0);
+      nzs[j] = 0;
+      if (v_atom < aggregate) {
+        int v_tile = config.tile_accessor(v_atom, p);
+        int row = config.tile_id(v_tile, p);
+        nzs[j] = config.atom_id(v_atom, row, v_tile, p);
+        keys[j] = TPT > 64 ? gbase + v_tile : v_tile;
+      }
     }
-    float v = prod;
+    int cols[kGmAhead];
+    float vals[kGmAhead];
+#pragma unroll
+    for (int j = 0; j < kGmAhead; ++j) {
+      cols[j] = ld_idx(indices + nzs[j]);
+      vals[j] = value_traits<V>::load1(values + nzs[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kGmAhead; ++j) {
+      const float xv = x[cols[j]];
+      prods[j] = base0 + j * TPT + rank < aggregate ? vals[j] * xv : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kGmAhead; ++j) {
+    if (base0 + j * TPT >= trip) break;
+    const int key = keys[j];
+    float v = prods[j];
     int head, tail;
     if constexpr (TPT <= 64) {
       int kp = __shfl_up(key, 1, TPT);
@@ -275,6 +301,7 @@ __global__ __launch_bounds__(TPB) void spmv_group_mapped_kernel(
       if (tail && key < TPB) s_acc[key] = s_acc[key] + v;
     }
     p.sync();
+    }
   }
   if (rank < config.get_length(p)) {
     int row = storage.tiles_indices[gbase + rank];

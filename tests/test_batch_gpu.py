@@ -38,7 +38,8 @@ SPEC = [(0, 1, "merge_path_flat", 4), (1, 1, "merge_path_flat", 8), (2, 1, "merg
         (2, 64, "merge_path_flat", 0), (0, 100, "merge_path_flat", 16), (7, 1, "merge_path_flat", 8),
         (7, 3, "merge_path_flat", 0), (6, 1, "thread_mapped", 0), (3, 1, "group_mapped", 0),
         (6, 1, "group_mapped", 64), (1, 1, "group_mapped", 2), (7, 1, "group_mapped", 0),
-        (4, 1, "group_mapped", 16), (6, 32, "merge_path_flat", 4), (5, 300, "merge_path_flat", 8)]
+        (4, 1, "group_mapped", 16), (6, 32, "merge_path_flat", 4), (5, 300, "merge_path_flat", 8),
+        (2, 65, "merge_path_flat", 32), (0, 129, "merge_path_flat", 0), (3, 40, "merge_path_flat", 32)]
 
 
 def _items(vt="f32", seed=0):

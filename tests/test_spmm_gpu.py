@@ -81,7 +81,7 @@ def test_spmm_leading_dimension(schedule, k, pad):
     assert np.all(np.abs(C - C64) <= REL * A64 + 1e-30)
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 7, 16, 33, 64, 300])
+@pytest.mark.parametrize("k", [1, 2, 3, 7, 16, 33, 64, 65, 129, 300])
 @pytest.mark.parametrize("ipt", [4, 8, 16])
 def test_spmm_merge_path_tile_sizes(ipt, k):
     """Merge-path SpMM tiles of 4, 8 or 16 steps per range, lane groups of

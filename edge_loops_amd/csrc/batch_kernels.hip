@@ -442,14 +442,18 @@ __global__ __launch_bounds__(256) void batch_spmv_fixup_kernel(const char* __res
   }
 }
 
-// SpMM tile fix-up across the batch: one lane per global tile slot probes
-// its tile; a row with one tile of carries is finished by that lane, the
-// others by its wavefront, one after another.
+// SpMM tile fix-up across the batch: kBatchFixLanes lanes per global tile
+// slot probe its tile; a row with one tile of carries is finished by them
+// (columns split between the lanes), the others by the wavefront, one after
+// another.
+constexpr int kBatchFixLanes = 4;
 __global__ __launch_bounds__(256) void batch_spmm_fixup_kernel(const char* __restrict__ table) {
   const batch_header* h = reinterpret_cast<const batch_header*>(table);
   const batch_item_dev* items =
       reinterpret_cast<const batch_item_dev*>(table + h->items_off);
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long s = gl / kBatchFixLanes;
+  const int sub = (int)(gl % kBatchFixLanes);
   int item = 0, r = -1;
   long long t = 0, t_end = 0;
   if (s < h->n_units) {
@@ -460,9 +464,10 @@ __global__ __launch_bounds__(256) void batch_spmm_fixup_kernel(const char* __res
     r = spmm_fixup_probe(it.offsets, it.rows, it.ti, t, it.sx, &t_end);
   }
   const bool alone = r >= 0 && t_end == t + 1;
-  if (alone) spmm_fixup_short(items[item].k, t, t_end, r, items[item].sx, items[item].y,
-                              items[item].ldy);
-  unsigned long long todo = __ballot(r >= 0 && !alone);
+  if (alone)
+    spmm_fixup_short_part(items[item].k, t, t_end, r, items[item].sx, items[item].y,
+                          items[item].ldy, sub, kBatchFixLanes);
+  unsigned long long todo = __ballot(r >= 0 && !alone && sub == 0);
   while (todo) {
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1;
@@ -966,7 +971,8 @@ hipError_t batch_launch(const void* dev_table, const loops_batch_info_t* info,
     if (e != hipSuccess) return e;
   }
   if (info->spmm_units > 0) {
-    batch_spmm_fixup_kernel<<<(unsigned)((info->spmm_units + 255) / 256), 256, 0, s>>>(t);
+    batch_spmm_fixup_kernel<<<(unsigned)((info->spmm_units * kBatchFixLanes + 255) / 256), 256,
+                              0, s>>>(t);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

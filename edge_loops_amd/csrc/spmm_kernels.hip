@@ -239,25 +239,28 @@ __global__ __launch_bounds__(TPB) void spmm_group_mapped_kernel(
   }
 }
 
-// One wavefront per 64 consecutive tiles: every lane probes its tile, then
-// the wavefront finishes the rows found, one after another.
+// One wavefront per 64 / L consecutive tiles, L lanes per tile
+// (spmm_fixup_lanes): the lanes of a tile probe it; a row whose carries lie
+// in one tile is finished by them, 4 columns per lane; the other rows by
+// the wavefront, one after another.
 __global__ __launch_bounds__(256) void spmm_fixup_kernel(const int* __restrict__ offsets,
                                                          int rows, int k, long long ti,
-                                                         long long tiles, spmm_ws ws,
+                                                         long long tiles, int L, spmm_ws ws,
                                                          float* __restrict__ C, long long ldc) {
-  const long long t0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) & ~63LL;
-  const long long t = t0 + (threadIdx.x & 63);
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const long long t0 = wave * (64 / L);
+  const int sub = lane & (L - 1);
+  const long long t = t0 + lane / L;
   long long t_end = 0;
   const int r = t < tiles ? spmm_fixup_probe(offsets, rows, ti, t, ws, &t_end) : -1;
-  // rows with one tile of carries by their own lane, the others by the
-  // wavefront, one after another
   const bool alone = r >= 0 && t_end == t + 1;
-  if (alone) spmm_fixup_short(k, t, t_end, r, ws, C, ldc);
-  unsigned long long todo = __ballot(r >= 0 && !alone);
+  if (alone) spmm_fixup_short_part(k, t, t_end, r, ws, C, ldc, sub, L);
+  unsigned long long todo = __ballot(r >= 0 && !alone && sub == 0);
   while (todo) {
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1;
-    spmm_fixup_row(k, t0 + src, __shfl(t_end, src, 64), __shfl(r, src, 64), ws, C, ldc);
+    spmm_fixup_row(k, t0 + src / L, __shfl(t_end, src, 64), __shfl(r, src, 64), ws, C, ldc);
   }
 }
 
@@ -330,8 +333,10 @@ hipError_t mp_spmm_t(const csr_args& A, int k, const float* B, long long ldb,
   e = record(ev1, s);
   if (e != hipSuccess) return e;
   if (tiles > 1) {
-    spmm_fixup_kernel<<<dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, s>>>(
-        A.offsets, A.rows, k, ti, tiles, w, C, ldc);
+    const int L = spmm_fixup_lanes(k);
+    const long long lanes = (tiles + 64 / L - 1) / (64 / L) * 64;
+    spmm_fixup_kernel<<<dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s>>>(
+        A.offsets, A.rows, k, ti, tiles, L, w, C, ldc);
     e = hipGetLastError();
   }
   return e;

@@ -385,20 +385,22 @@ __device__ __forceinline__ int spmm_fixup_probe(const int* __restrict__ offsets,
   return r;
 }
 
-// spmm_fixup_short: a row whose carries lie in one tile (t_end == t + 1),
-// finished by its own lane: C = (0 + carry[t]) + head[t_end] per column,
-// the value spmm_fixup_row gives it (its other partitions add +0).
-__device__ __forceinline__ void spmm_fixup_short(int k, long long t, long long t_end, int r,
-                                                 const spmm_ws& ws, float* __restrict__ C,
-                                                 long long ldc) {
+// spmm_fixup_short_part: a row whose carries lie in one tile (t_end == t +
+// 1), finished by L lanes (lane sub takes columns 4 sub, 4 (sub + L), ...
+// where the slots allow 16-byte loads, else columns sub, sub + L, ...):
+// C = (0 + carry[t]) + head[t_end] per column, the value spmm_fixup_row
+// gives it (its other partitions add +0).  spmm_fixup_short: one lane.
+__device__ __forceinline__ void spmm_fixup_short_part(int k, long long t, long long t_end,
+                                                      int r, const spmm_ws& ws,
+                                                      float* __restrict__ C, long long ldc,
+                                                      int sub, int L) {
   const float* __restrict__ cr = ws.carry + t * k;
   const float* __restrict__ hr = ws.head + t_end * k;
   float* __restrict__ out = C + (long long)r * ldc;
-  int c = 0;
   if (((reinterpret_cast<uintptr_t>(cr) | reinterpret_cast<uintptr_t>(hr) |
         (uintptr_t)k) & 3) == 0 &&
       ((reinterpret_cast<uintptr_t>(cr) | reinterpret_cast<uintptr_t>(hr)) & 15) == 0) {
-    for (; c < k; c += 4) {
+    for (int c = 4 * sub; c < k; c += 4 * L) {
       const f32x4 a = *reinterpret_cast<const f32x4*>(cr + c);
       const f32x4 h = *reinterpret_cast<const f32x4*>(hr + c);
       out[c] = (0.f + a.x) + h.x;
@@ -406,8 +408,22 @@ __device__ __forceinline__ void spmm_fixup_short(int k, long long t, long long t
       out[c + 2] = (0.f + a.z) + h.z;
       out[c + 3] = (0.f + a.w) + h.w;
     }
+    return;
   }
-  for (; c < k; ++c) out[c] = (0.f + cr[c]) + hr[c];
+  for (int c = sub; c < k; c += L) out[c] = (0.f + cr[c]) + hr[c];
+}
+__device__ __forceinline__ void spmm_fixup_short(int k, long long t, long long t_end, int r,
+                                                 const spmm_ws& ws, float* __restrict__ C,
+                                                 long long ldc) {
+  spmm_fixup_short_part(k, t, t_end, r, ws, C, ldc, 0, 1);
+}
+
+// Lanes per tile of the fix-up kernel: one per 4 columns, a power of two,
+// at most 16.
+__host__ __device__ inline int spmm_fixup_lanes(int k) {
+  int L = 1;
+  while (L < 16 && 4 * L < k) L <<= 1;
+  return L;
 }
 
 // spmm_fixup_row: the calling wavefront splits its 64 lanes into kp column
